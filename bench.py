@@ -1,0 +1,224 @@
+"""Benchmark: converged eigenpairs/s of the GCG eigensolver on one MI355X.
+
+Workload (default): the 3D 7-point problem on a 128^3 interior grid,
+N = 2,097,152, A = 6I - Adj, B = (12I + Adj)/18, CCS FP64, the nev = 100
+smallest pairs to 1e-8 relative residual with block size 120
+(nevInit = nevMax = 220).  One step = one complete GCG solve with the
+matrices resident in HBM (upload timed separately).
+
+Two paths are run on the same problem:
+  fp64  -- the reference-equivalent path: every operation FP64 on the GPU,
+           the reference's default autoshift (shift = 1);
+  mixed -- FP32 device BlockPCG in the MultiLinearSolver slot, FP64
+           everywhere else, shift = 0 (the reference forbids autoshift
+           with a user solver).
+`value` is the mixed path's whole-job eigenpairs/s; the fp64 path's is
+reported beside it.
+
+The roofline object is for the dominant kernel class of the mixed path,
+timed with HIP events on the library's stream over the timed region.
+The CPU baseline runs the reference GCGE (compiled from its sources in
+oracle/_ref, OpenMP build) on the same workload for a bounded sample and
+extrapolates with the GPU run's iteration count.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import subprocess
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFS = 78.6
+FP32_PEAK_TFS = 157.3
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def make_problem(name: str):
+    from gcge_amd import problems as P
+    if name == "cfg2":
+        return P.config2(), 128, 3
+    if name == "cfg1":
+        return P.config1(), 128, 2
+    if name.startswith("3d_"):
+        n = int(name[3:])
+        return P.stencil_problem(n, 3, 100, 120, 220, 220, name=f"3d_{n}"), n, 3
+    raise ValueError(name)
+
+
+def run_path(prob, n, dim, mode, steps, warmup, mats, want_check=True):
+    from gcge_amd import abi, problems as P, solver as S
+    lib = abi.load()
+    shift = 1 if mode == "fp64" else 0
+    kw = dict(shift=shift, mode=mode, rand="device", seed=1, mats=mats, print_level=1)
+    for w in range(warmup):
+        log(f"{mode}: warmup {w + 1}/{warmup}")
+        S.solve_problem(prob, **kw)
+    lib.gcge_hip_profile_reset()
+    lib.gcge_hip_profile_enable(1)
+    res, secs = None, []
+    for k in range(steps):
+        last = (k == steps - 1)
+        res = S.solve_problem(prob, want_vectors=(want_check and last), **kw)
+        secs.append(res.seconds)
+        log(f"{mode}: step {k + 1}/{steps}: {res.seconds:.3f} s, iters {res.num_iter}, "
+            f"nconv {res.nev_conv}, bpcg {res.bpcg_iters}")
+    lib.gcge_hip_profile_enable(0)
+    prof = {}
+    import ctypes as C
+    for kind, name in enumerate(("spmm64", "spmm32", "gram", "lincomb", "blas1")):
+        la, ms, by, fl = C.c_int64(), C.c_double(), C.c_double(), C.c_double()
+        lib.gcge_hip_profile_read(kind, C.byref(la), C.byref(ms), C.byref(by), C.byref(fl))
+        if la.value:
+            prof[name] = {"launches": la.value, "total_ms": ms.value,
+                          "avg_us": 1e3 * ms.value / la.value,
+                          "bytes": by.value, "flops": fl.value,
+                          "GBs": by.value / (ms.value * 1e-3) / 1e9 if ms.value else 0.0,
+                          "TFs": fl.value / (ms.value * 1e-3) / 1e12 if ms.value else 0.0}
+    k = res.nev_conv
+    out = {"seconds": secs, "num_iter": res.num_iter, "nev_conv": k,
+           "bpcg_iters": res.bpcg_iters, "phases": res.phases, "kernels": prof}
+    if want_check and res.evec is not None:
+        exact = P.analytic_eigenvalues(n, dim, k)
+        out["max_rel_eig_err_vs_analytic"] = float(np.max(np.abs(res.eval[:k] - exact) / exact))
+        X = res.evec[:, :k]
+        lam = res.eval[:k]
+        R = prob.A @ X - (prob.B @ X) * lam[None, :]
+        rr = np.sqrt(np.einsum("ij,ij->j", R, R)) / np.abs(lam)
+        out["max_rel_residual"] = float(rr.max())
+        out["eval_first_last"] = [float(lam[0]), float(lam[min(prob.nev, k) - 1])]
+        del R, X
+    return out
+
+
+def cpu_baseline(prob, gpu_iters, nev_conv, iters_sample):
+    """Reference GCGE (OpenMP build) on the same workload for a bounded
+    number of outer iterations, in a child process; extrapolated."""
+    refso = os.path.join(ROOT, "oracle", "_ref", "libgcge_ref_omp.so")
+    if not os.path.exists(refso):
+        return None, "oracle/_ref/libgcge_ref_omp.so not built"
+    cores = min(os.cpu_count() or 1, 16)
+    env = dict(os.environ, OMP_NUM_THREADS=str(cores), MKL_NUM_THREADS="1")
+    cmd = [sys.executable, os.path.join(ROOT, "oracle", "ref_sample.py"), refso,
+           str(iters_sample)]
+    log(f"cpu baseline: reference GCGE, {cores} threads, {iters_sample} outer iteration(s)")
+    try:
+        p = subprocess.run(cmd, env=env, capture_output=True, text=True, timeout=900)
+    except subprocess.TimeoutExpired:
+        return None, "reference sample timed out"
+    if p.returncode != 0:
+        return None, f"reference sample failed: {p.stderr[-300:]}"
+    r = json.loads(p.stdout.strip().splitlines()[-1])
+    t_init = r["phases"]["initX"]
+    t_sample = r["seconds"]
+    per_iter = (t_sample - t_init) / max(r["num_iter"], 1)
+    t_full = t_init + per_iter * gpu_iters
+    val = nev_conv / t_full
+    return {"value": val, "unit": "eigenpairs/s", "cores": cores, "kind": "reference",
+            "sample": (f"reference GCGE (OpenMP build, MKL BLAS/LAPACK 1 thread) on the same "
+                       f"128^3 problem: InitializeX + {r['num_iter']} outer iteration(s) in "
+                       f"{t_sample:.1f} s (initX {t_init:.1f} s, {per_iter:.1f} s/iteration), "
+                       f"extrapolated to the GPU run's {gpu_iters} iterations"),
+            "seconds_extrapolated": t_full}, None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1)
+    ap.add_argument("--warmup", type=int, default=0)
+    ap.add_argument("--config", default="cfg2")
+    ap.add_argument("--modes", default="fp64,mixed")
+    ap.add_argument("--cpu-baseline", default="on", choices=("on", "off"))
+    ap.add_argument("--cpu-iters", type=int, default=1)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+
+    from gcge_amd import abi, problems as P, solver as S
+    abi.load().gcge_host_set_print_target(1)  # monitor lines to stderr
+    t0 = time.perf_counter()
+    prob, n, dim = make_problem(args.config)
+    log(f"problem {prob.name}: N={prob.n}, nnz={prob.A.nnz}, gen {time.perf_counter() - t0:.1f}s")
+    t0 = time.perf_counter()
+    mats = (S.DeviceMatrix(prob.A, fp32=True), S.DeviceMatrix(prob.B, fp32=True))
+    t_upload = time.perf_counter() - t0
+    log(f"upload {t_upload:.2f}s")
+    # load the kernels and the dense eigensolver once (not a step)
+    pp = P.stencil_problem(12, 3, 30)
+    S.solve_problem(pp, shift=1, rand="device", eig="device")
+    S.solve_problem(pp, shift=0, mode="mixed", rand="device")
+
+    results = {}
+    for mode in args.modes.split(","):
+        results[mode] = run_path(prob, n, dim, mode, args.steps, args.warmup, mats)
+
+    head = "mixed" if "mixed" in results else list(results)[0]
+    hr = results[head]
+    secs = np.array(hr["seconds"])
+    value = hr["nev_conv"] / secs.mean()
+    # dominant kernel class of the headline path
+    kern = hr["kernels"]
+    dom = max(kern, key=lambda k: kern[k]["total_ms"]) if kern else None
+    roof = None
+    if dom:
+        kd = kern[dom]
+        bound_mfma = dom in ("gram", "lincomb") and kd["flops"] / max(kd["bytes"], 1) > 9.8
+        if bound_mfma:
+            roof = {"bound": "mfma", "kernel": dom, "achieved": kd["TFs"],
+                    "peak": FP64_PEAK_TFS, "unit": "TFLOP/s",
+                    "frac": kd["TFs"] / FP64_PEAK_TFS, "traffic": None}
+        else:
+            roof = {"bound": "hbm", "kernel": dom, "achieved": kd["GBs"],
+                    "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": kd["GBs"] / HBM_PEAK_GBS,
+                    "traffic": None, "avg_launch_us": kd["avg_us"],
+                    "algorithmic_bytes_per_launch": kd["bytes"] / kd["launches"]}
+    cpu, why = (None, "disabled")
+    if args.cpu_baseline == "on" and args.config == "cfg2":
+        cpu, why = cpu_baseline(prob, hr["num_iter"], hr["nev_conv"], args.cpu_iters)
+    line = {
+        "metric": "converged eigenpairs/s at 1e-8 residual, mixed vs FP64; p50/p99 solve latency",
+        "value": value, "unit": "eigenpairs/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": 1e3 * float(secs.mean()),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64+f32" if head == "mixed" else "f64", "data": "synthetic",
+        "config": {"workload": (f"{prob.name}: generalized eigenproblem, N={prob.n}, "
+                                f"nnz={prob.A.nnz} per matrix, nev={prob.nev}, "
+                                f"block {prob.block_size}, nevInit=nevMax={prob.nev_max}, "
+                                f"tol 1e-8 rel"),
+                   "path": head, "parallelism": "single GPU"},
+        "latency_ms": {"p50": 1e3 * float(np.percentile(secs, 50)),
+                       "p99": 1e3 * float(np.percentile(secs, 99))},
+        "paths": {m: {"eigenpairs_per_s": r["nev_conv"] / float(np.mean(r["seconds"])),
+                      "seconds": r["seconds"], "num_iter": r["num_iter"],
+                      "nev_conv": r["nev_conv"], "bpcg_iters": r["bpcg_iters"],
+                      "max_rel_eig_err_vs_analytic": r.get("max_rel_eig_err_vs_analytic"),
+                      "max_rel_residual": r.get("max_rel_residual"),
+                      "phases_s": r["phases"], "kernels": r["kernels"]}
+                  for m, r in results.items()},
+        "upload_s": t_upload,
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    if "fp64" in results and "mixed" in results:
+        line["speedup_mixed_vs_fp64"] = (np.mean(results["fp64"]["seconds"]) /
+                                         np.mean(results["mixed"]["seconds"]))
+    if cpu is None:
+        line["cpu_baseline_note"] = why
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

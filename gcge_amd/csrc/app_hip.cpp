@@ -1,0 +1,765 @@
+// The HIP "app" of the GCG eigensolver: fills the operation table with
+// device implementations (the role app/app_ccs.c:213-249 plays for the
+// CPU).  Multivectors and matrices live in device memory; only the small
+// coefficient and result matrices cross to the host, as the table's
+// signatures require (src/ops.h:78-103).
+//
+// Slot semantics follow app/app_ccs.c:50-150 (sparse products),
+// app/app_lapack.c:230-534 (multivector algebra) and
+// src/ops_multi_vec.c:351-411 (MultiVecQtAP).
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "app_hip_internal.h"
+#include "gcge_hip.h"
+#include "kernels.h"
+
+namespace gcge {
+
+// ------------------------------------------------------------------ errors
+static std::mutex g_err_mu;
+static std::string g_err;
+static int g_err_code = 0;
+
+void fail(const char *what, const char *file, int line)
+{
+    {
+        std::lock_guard<std::mutex> lk(g_err_mu);
+        char buf[512];
+        snprintf(buf, sizeof(buf), "gcge_hip: %s (%s:%d)", what, file, line);
+        g_err = buf;
+        g_err_code = 1;
+    }
+    fprintf(stderr, "%s\n", g_err.c_str());
+    fflush(stderr);
+    abort();
+}
+
+void hip_check(hipError_t e, const char *expr, const char *file, int line)
+{
+    if (e == hipSuccess) return;
+    char buf[400];
+    snprintf(buf, sizeof(buf), "%s failed: %s", expr, hipGetErrorString(e));
+    fail(buf, file, line);
+}
+
+// ----------------------------------------------------------------- context
+Ctx &ctx()
+{
+    static Ctx *c = [] {
+        Ctx *x = new Ctx();
+        HIPCHECK(hipGetDevice(&x->device));
+        HIPCHECK(hipStreamCreateWithFlags(&x->stream, hipStreamNonBlocking));
+        return x;
+    }();
+    return *c;
+}
+
+double *Ctx::scratch(int64_t ndouble)
+{
+    if (ndouble > scratch_cap) {
+        sync();
+        if (d_scratch) HIPCHECK(hipFree(d_scratch));
+        int64_t cap = ndouble + ndouble / 4 + 1024;
+        HIPCHECK(hipMalloc(reinterpret_cast<void **>(&d_scratch), cap * sizeof(double)));
+        scratch_cap = cap;
+    }
+    return d_scratch;
+}
+
+double *Ctx::small(int64_t ndouble)
+{
+    if (ndouble > small_cap) {
+        sync();
+        if (d_small) HIPCHECK(hipFree(d_small));
+        int64_t cap = ndouble + 4096;
+        HIPCHECK(hipMalloc(reinterpret_cast<void **>(&d_small), cap * sizeof(double)));
+        small_cap = cap;
+    }
+    return d_small;
+}
+
+double *Ctx::pinned(int64_t ndouble)
+{
+    if (ndouble > pinned_cap) {
+        // the stream may still read the old staging buffer
+        HIPCHECK(hipStreamSynchronize(stream));
+        if (h_pinned) HIPCHECK(hipHostFree(h_pinned));
+        int64_t cap = ndouble + 4096;
+        HIPCHECK(hipHostMalloc(reinterpret_cast<void **>(&h_pinned), cap * sizeof(double),
+                               hipHostMallocDefault));
+        pinned_cap = cap;
+    }
+    return h_pinned;
+}
+
+void Ctx::sync() { HIPCHECK(hipStreamSynchronize(stream)); }
+
+// ---------------------------------------------------------------- profiler
+hipEvent_t Profiler::begin(int kind, double b, double f, hipStream_t s)
+{
+    if (!on) return nullptr;
+    auto take = [&]() {
+        hipEvent_t e;
+        if (!pool.empty()) {
+            e = pool.back();
+            pool.pop_back();
+        } else {
+            HIPCHECK(hipEventCreate(&e));
+        }
+        return e;
+    };
+    Rec r;
+    r.a = take();
+    r.b = take();
+    r.kind = kind;
+    r.bytes = b;
+    r.flops = f;
+    HIPCHECK(hipEventRecord(r.a, s));
+    pending.push_back(r);
+    return r.b;
+}
+
+void Profiler::end(hipEvent_t e, hipStream_t s)
+{
+    if (e) HIPCHECK(hipEventRecord(e, s));
+    if (pending.size() > 4096) collect();
+}
+
+void Profiler::collect()
+{
+    for (Rec &r : pending) {
+        HIPCHECK(hipEventSynchronize(r.b));
+        float t = 0.f;
+        HIPCHECK(hipEventElapsedTime(&t, r.a, r.b));
+        launches[r.kind] += 1;
+        ms[r.kind] += t;
+        bytes[r.kind] += r.bytes;
+        flops[r.kind] += r.flops;
+        pool.push_back(r.a);
+        pool.push_back(r.b);
+    }
+    pending.clear();
+}
+
+void Profiler::reset()
+{
+    collect();
+    for (int k = 0; k < PROF_NKIND; ++k) {
+        launches[k] = 0;
+        ms[k] = bytes[k] = flops[k] = 0.0;
+    }
+}
+
+// ------------------------------------------------------------- multivector
+MV *mv_of(void **p, const char *where)
+{
+    MV *m = reinterpret_cast<MV *>(p);
+    if (m == nullptr || m->magic != MV::kMagic) {
+        char buf[128];
+        snprintf(buf, sizeof(buf), "%s: not a HIP multivector handle", where);
+        fail(buf, __FILE__, __LINE__);
+    }
+    return m;
+}
+
+Mat *mat_of(void *p, const char *where)
+{
+    Mat *m = static_cast<Mat *>(p);
+    if (m == nullptr || m->magic != Mat::kMagic) {
+        char buf[128];
+        snprintf(buf, sizeof(buf), "%s: not a HIP matrix handle", where);
+        fail(buf, __FILE__, __LINE__);
+    }
+    return m;
+}
+
+MV *mv_create(int64_t nrows, int64_t ncols)
+{
+    Ctx &c = ctx();
+    MV *m = new MV();
+    m->nrows = nrows;
+    m->ncols = ncols;
+    m->ldd = nrows;
+    m->owner = true;
+    const size_t bytes = (size_t)(nrows * (ncols > 0 ? ncols : 1)) * sizeof(double);
+    HIPCHECK(hipMalloc(reinterpret_cast<void **>(&m->d), bytes));
+    HIPCHECK(hipMemsetAsync(m->d, 0, bytes, c.stream));
+    return m;
+}
+
+void mv_destroy(MV *m)
+{
+    if (!m) return;
+    if (m->owner && m->d) {
+        ctx().sync();
+        HIPCHECK(hipFree(m->d));
+    }
+    m->magic = 0;
+    delete m;
+}
+
+static void check_range(const MV *m, int s, int e, const char *where)
+{
+    if (s < 0 || e < s || e > m->ncols) {
+        char buf[160];
+        snprintf(buf, sizeof(buf), "%s: column range [%d,%d) outside [0,%lld)", where, s, e,
+                 (long long)m->ncols);
+        fail(buf, __FILE__, __LINE__);
+    }
+}
+
+}  // namespace gcge
+
+using namespace gcge;
+
+namespace {
+
+// ------------------------------------------------------------------- slots
+void h_MultiVecCreateByMat(void ***des, int num_vec, void *src_mat, OPS *)
+{
+    Mat *A = mat_of(src_mat, "MultiVecCreateByMat");
+    *des = reinterpret_cast<void **>(mv_create(A->n, num_vec));
+}
+
+void h_MultiVecCreateByVec(void ***des, int num_vec, void *src_vec, OPS *)
+{
+    MV *s = mv_of(reinterpret_cast<void **>(src_vec), "MultiVecCreateByVec");
+    *des = reinterpret_cast<void **>(mv_create(s->nrows, num_vec));
+}
+
+void h_MultiVecCreateByMultiVec(void ***des, int num_vec, void **src, OPS *)
+{
+    MV *s = mv_of(src, "MultiVecCreateByMultiVec");
+    *des = reinterpret_cast<void **>(mv_create(s->nrows, num_vec));
+}
+
+void h_MultiVecDestroy(void ***des, int, OPS *)
+{
+    if (*des) mv_destroy(mv_of(*des, "MultiVecDestroy"));
+    *des = nullptr;
+}
+
+void h_GetVecFromMultiVec(void **multi_vec, int c, void **vec, OPS *)
+{
+    MV *m = mv_of(multi_vec, "GetVecFromMultiVec");
+    check_range(m, c, c + 1, "GetVecFromMultiVec");
+    MV *v = new MV();
+    v->nrows = m->nrows;
+    v->ncols = 1;
+    v->ldd = m->ldd;
+    v->d = m->d + (int64_t)c * m->ldd;
+    v->owner = false;
+    *vec = v;
+}
+
+void h_RestoreVecForMultiVec(void **, int, void **vec, OPS *)
+{
+    if (*vec) mv_destroy(mv_of(reinterpret_cast<void **>(*vec), "RestoreVecForMultiVec"));
+    *vec = nullptr;
+}
+
+void h_MultiVecView(void **x, int start, int end, OPS *ops)
+{
+    MV *m = mv_of(x, "MultiVecView");
+    check_range(m, start, end, "MultiVecView");
+    const int nc = end - start;
+    std::vector<double> h((size_t)m->nrows * (nc > 0 ? nc : 1));
+    Ctx &c = ctx();
+    for (int k = 0; k < nc; ++k)
+        HIPCHECK(hipMemcpyAsync(h.data() + (size_t)m->nrows * k,
+                                m->d + (int64_t)(start + k) * m->ldd,
+                                sizeof(double) * m->nrows, hipMemcpyDeviceToHost, c.stream));
+    c.sync();
+    for (int64_t r = 0; r < m->nrows; ++r) {
+        for (int k = 0; k < nc; ++k) ops->Printf("%6.4e\t", h[(size_t)m->nrows * k + r]);
+        ops->Printf("\n");
+    }
+}
+
+// y[:, s1:e1] = mat * x[:, s0:e0]; mat NULL copies (app/app_ccs.c:50-139)
+void h_MatDotMultiVec(void *mat, void **x, void **y, int *start, int *end, OPS *)
+{
+    MV *X = mv_of(x, "MatDotMultiVec x"), *Y = mv_of(y, "MatDotMultiVec y");
+    const int nc = end[0] - start[0];
+    if (nc != end[1] - start[1]) fail("MatDotMultiVec: ranges differ in length", __FILE__, __LINE__);
+    check_range(X, start[0], end[0], "MatDotMultiVec x");
+    check_range(Y, start[1], end[1], "MatDotMultiVec y");
+    if (nc == 0) return;
+    Ctx &c = ctx();
+    const double *xs = X->d + (int64_t)start[0] * X->ldd;
+    double *ys = Y->d + (int64_t)start[1] * Y->ldd;
+    if (mat == nullptr) {
+        copy_cols(Y->nrows, nc, xs, X->ldd, ys, Y->ldd, c.stream);
+        return;
+    }
+    Mat *A = mat_of(mat, "MatDotMultiVec");
+    if (A->n != X->nrows || A->n != Y->nrows)
+        fail("MatDotMultiVec: matrix and vector sizes differ", __FILE__, __LINE__);
+    // algorithmic bytes: CCS arrays once, X read once, Y written once
+    const double bytes = (double)A->sell.nnz_true * 12.0 + 4.0 * (double)(A->n + 1) +
+                         2.0 * 8.0 * (double)nc * (double)A->n;
+    hipEvent_t ev = c.prof.begin(PROF_SPMM64, bytes, 2.0 * (double)A->sell.nnz_true * nc,
+                                 c.stream);
+    spmm(A->sell, xs, X->ldd, ys, Y->ldd, nc, c.stream);
+    c.prof.end(ev, c.stream);
+    c.stats.spmm_calls++;
+    c.stats.spmm_cols += nc;
+}
+
+// y = alpha x + beta y over column ranges (app/app_lapack.c:334-395)
+void h_MultiVecAxpby(double alpha, void **x, double beta, void **y, int *start, int *end,
+                     OPS *)
+{
+    MV *Y = mv_of(y, "MultiVecAxpby y");
+    const int nc = end[1] - start[1];
+    if (nc != end[0] - start[0]) fail("MultiVecAxpby: ranges differ in length", __FILE__, __LINE__);
+    check_range(Y, start[1], end[1], "MultiVecAxpby y");
+    if (nc == 0 || Y->nrows == 0) return;
+    const double *xs = nullptr;
+    int64_t ldx = 0;
+    if (x != nullptr) {
+        MV *X = mv_of(x, "MultiVecAxpby x");
+        check_range(X, start[0], end[0], "MultiVecAxpby x");
+        if (X->nrows != Y->nrows) fail("MultiVecAxpby: row counts differ", __FILE__, __LINE__);
+        xs = X->d + (int64_t)start[0] * X->ldd;
+        ldx = X->ldd;
+    }
+    axpby(Y->nrows, nc, alpha, xs, ldx, beta, Y->d + (int64_t)start[1] * Y->ldd, Y->ldd,
+          ctx().stream);
+}
+
+// y = x coef + y diag(beta) (app/app_lapack.c:463-534)
+void h_MultiVecLinearComb(void **x, void **y, int, int *start, int *end, double *coef, int ldc,
+                          double *beta, int incb, OPS *)
+{
+    MV *Y = mv_of(y, "MultiVecLinearComb y");
+    const int nr = end[0] - start[0], nc = end[1] - start[1];
+    if (nr == 0 || nc == 0 || Y->nrows == 0) return;
+    check_range(Y, start[1], end[1], "MultiVecLinearComb y");
+    Ctx &c = ctx();
+    double *ys = Y->d + (int64_t)start[1] * Y->ldd;
+    const bool have_x = (x != nullptr && coef != nullptr);
+    if (!have_x) {
+        if (beta == nullptr) return;
+        if (incb == 0) {
+            if (*beta != 1.0) scale_cols(Y->nrows, nc, nullptr, *beta, ys, Y->ldd, c.stream);
+        } else {
+            double *h = c.pinned(nc);
+            for (int k = 0; k < nc; ++k) h[k] = beta[(int64_t)k * incb];
+            double *d = c.small(nc);
+            HIPCHECK(hipMemcpyAsync(d, h, sizeof(double) * nc, hipMemcpyHostToDevice, c.stream));
+            scale_cols(Y->nrows, nc, d, 0.0, ys, Y->ldd, c.stream);
+            c.sync();  // the pinned staging buffer is reused by the next call
+        }
+        return;
+    }
+    MV *X = mv_of(x, "MultiVecLinearComb x");
+    check_range(X, start[0], end[0], "MultiVecLinearComb x");
+    if (X->nrows != Y->nrows) fail("MultiVecLinearComb: row counts differ", __FILE__, __LINE__);
+    // stage coef (nr x nc, packed) and beta in one upload
+    const int64_t ncoef = (int64_t)nr * nc;
+    double *h = c.pinned(ncoef + nc);
+    for (int k = 0; k < nc; ++k)
+        memcpy(h + (int64_t)nr * k, coef + (int64_t)ldc * k, sizeof(double) * nr);
+    int beta_mode = 0;
+    double beta_scalar = 0.0;
+    if (beta != nullptr) {
+        if (incb == 0) {
+            beta_mode = 1;
+            beta_scalar = *beta;
+        } else {
+            beta_mode = 2;
+            for (int k = 0; k < nc; ++k) h[ncoef + k] = beta[(int64_t)k * incb];
+        }
+    }
+    double *d = c.small(ncoef + nc);
+    HIPCHECK(hipMemcpyAsync(d, h, sizeof(double) * (ncoef + (beta_mode == 2 ? nc : 0)),
+                            hipMemcpyHostToDevice, c.stream));
+    hipEvent_t ev = c.prof.begin(
+        PROF_LINCOMB, 8.0 * (double)Y->nrows * (nr + nc * (beta_mode ? 2.0 : 1.0)),
+        2.0 * (double)Y->nrows * nr * nc, c.stream);
+    lincomb(Y->nrows, nr, nc, X->d + (int64_t)start[0] * X->ldd, X->ldd, d, nr, beta_mode,
+            beta_scalar, d + ncoef, ys, Y->ldd, c.stream);
+    c.prof.end(ev, c.stream);
+    c.stats.lincomb_calls++;
+    c.stats.lincomb_flops += 2.0 * (double)Y->nrows * nr * nc;
+    c.sync();  // the pinned staging buffer is reused by the next call
+}
+
+// Inner products to a host array (app/app_lapack.c:299-321, 24-183)
+void inner_prod_impl(char nsd, MV *X, MV *Y, int *start, int *end, double *ip, int ldIP)
+{
+    const int nr = end[0] - start[0], nc = end[1] - start[1];
+    if (nr <= 0 || nc <= 0) return;
+    check_range(X, start[0], end[0], "MultiVecInnerProd x");
+    check_range(Y, start[1], end[1], "MultiVecInnerProd y");
+    if (X->nrows != Y->nrows) fail("MultiVecInnerProd: row counts differ", __FILE__, __LINE__);
+    Ctx &c = ctx();
+    const double *xs = X->d + (int64_t)start[0] * X->ldd;
+    const double *ys = Y->d + (int64_t)start[1] * Y->ldd;
+    if (nsd == 'D') {
+        if (nr != nc) fail("MultiVecInnerProd 'D': ranges differ", __FILE__, __LINE__);
+        double *part = c.scratch(dots_scratch(nc));
+        double *d = c.small(nc);
+        col_dots(X->nrows, nc, xs, X->ldd, ys, Y->ldd, d, part, c.stream);
+        double *h = c.pinned(nc);
+        HIPCHECK(hipMemcpyAsync(h, d, sizeof(double) * nc, hipMemcpyDeviceToHost, c.stream));
+        c.sync();
+        for (int k = 0; k < nc; ++k) ip[(int64_t)ldIP * k] = h[k];
+        c.stats.dot_calls++;
+        return;
+    }
+    const bool sym = (nsd == 'S');
+    if (sym && nr != nc) fail("MultiVecInnerProd 'S': block not square", __FILE__, __LINE__);
+    double *part = c.scratch(gram_scratch(X->nrows, nr, nc));
+    double *d = c.small((int64_t)nr * nc);
+    hipEvent_t ev = c.prof.begin(PROF_GRAM, 8.0 * (double)X->nrows * (nr + nc),
+                                 2.0 * (double)X->nrows * nr * nc, c.stream);
+    gram(X->nrows, nr, nc, xs, X->ldd, ys, Y->ldd, d, nr, sym, part, c.stream);
+    c.prof.end(ev, c.stream);
+    double *h = c.pinned((int64_t)nr * nc);
+    HIPCHECK(hipMemcpyAsync(h, d, sizeof(double) * nr * nc, hipMemcpyDeviceToHost, c.stream));
+    c.sync();
+    c.stats.gram_calls++;
+    c.stats.gram_flops += 2.0 * (double)X->nrows * nr * nc * (sym ? 0.5 : 1.0);
+    for (int j = 0; j < nc; ++j) {
+        if (sym) {
+            for (int i = j; i < nr; ++i) {
+                const double v = h[i + (int64_t)nr * j];
+                ip[i + (int64_t)ldIP * j] = v;
+                ip[j + (int64_t)ldIP * i] = v;
+            }
+        } else {
+            for (int i = 0; i < nr; ++i) ip[i + (int64_t)ldIP * j] = h[i + (int64_t)nr * j];
+        }
+    }
+}
+
+void h_MultiVecInnerProd(char nsdIP, void **x, void **y, int, int *start, int *end,
+                         double *ip, int ldIP, OPS *)
+{
+    inner_prod_impl(nsdIP, mv_of(x, "MultiVecInnerProd x"), mv_of(y, "MultiVecInnerProd y"),
+                    start, end, ip, ldIP);
+}
+
+// qAp = Q^T op(A) P with op(A) P left in mv_ws (src/ops_multi_vec.c:351-411)
+void h_MultiVecQtAP(char ntsA, char ntsdQAP, void **mvQ, void *matA, void **mvP, int is_vec,
+                    int *startQP, int *endQP, double *qAp, int ldQAP, void **mv_ws, OPS *ops)
+{
+    const int nr = endQP[0] - startQP[0], nc = endQP[1] - startQP[1];
+    if (nr <= 0 || nc <= 0) return;
+    int s[2], e[2];
+    if (matA == nullptr) {
+        if (ntsdQAP == 'T') {
+            s[0] = startQP[1]; e[0] = endQP[1];
+            s[1] = startQP[0]; e[1] = endQP[0];
+            h_MultiVecInnerProd('N', mvP, mvQ, is_vec, s, e, qAp, ldQAP, ops);
+        } else {
+            h_MultiVecInnerProd(ntsdQAP, mvQ, mvP, is_vec, startQP, endQP, qAp, ldQAP, ops);
+        }
+        return;
+    }
+    // op(A) = A: the backend requires symmetric matrices ('T' == 'N')
+    (void)ntsA;
+    s[0] = startQP[1]; e[0] = endQP[1];
+    s[1] = 0; e[1] = nc;
+    h_MatDotMultiVec(matA, mvP, mv_ws, s, e, ops);
+    if (ntsdQAP == 'T') {
+        s[0] = 0; e[0] = nc;
+        s[1] = startQP[0]; e[1] = endQP[0];
+        h_MultiVecInnerProd('N', mv_ws, mvQ, is_vec, s, e, qAp, ldQAP, ops);
+    } else {
+        s[0] = startQP[0]; e[0] = endQP[0];
+        s[1] = 0; e[1] = nc;
+        h_MultiVecInnerProd(ntsdQAP, mvQ, mv_ws, is_vec, s, e, qAp, ldQAP, ops);
+    }
+}
+
+void h_MultiVecSetRandomValue(void **x, int start, int end, OPS *)
+{
+    MV *m = mv_of(x, "MultiVecSetRandomValue");
+    check_range(m, start, end, "MultiVecSetRandomValue");
+    const int nc = end - start;
+    if (nc <= 0 || m->nrows == 0) return;
+    Ctx &c = ctx();
+    if (c.rand_mode == GCGE_HIP_RAND_DEVICE) {
+        random_uniform(m->nrows, nc, m->d + (int64_t)start * m->ldd, m->ldd, c.rand_seed,
+                       c.rand_counter, c.stream);
+        c.rand_counter += (uint64_t)m->nrows * (uint64_t)nc;
+        return;
+    }
+    // libc rand(), column by column (app/app_lapack.c:322-333)
+    const int64_t n = m->nrows;
+    double *h = c.pinned(n);
+    for (int k = 0; k < nc; ++k) {
+        for (int64_t r = 0; r < n; ++r) h[r] = (double)rand() / ((double)RAND_MAX + 1.0);
+        HIPCHECK(hipMemcpyAsync(m->d + (int64_t)(start + k) * m->ldd, h, sizeof(double) * n,
+                                hipMemcpyHostToDevice, c.stream));
+        c.sync();
+    }
+}
+
+void h_MatView(void *mat, OPS *ops)
+{
+    Mat *A = mat_of(mat, "MatView");
+    ops->Printf("HIP SELL-64 matrix: %lld x %lld, nnz %lld (stored %lld)\n", (long long)A->n,
+                (long long)A->n, (long long)A->sell.nnz_true, (long long)A->sell.nnz);
+}
+
+// single vectors are one-column multivectors
+void h_VecCreateByMat(void **v, void *m, OPS *ops)
+{
+    h_MultiVecCreateByMat(reinterpret_cast<void ***>(v), 1, m, ops);
+}
+void h_VecCreateByVec(void **v, void *s, OPS *ops)
+{
+    h_MultiVecCreateByVec(reinterpret_cast<void ***>(v), 1, s, ops);
+}
+void h_VecDestroy(void **v, OPS *ops) { h_MultiVecDestroy(reinterpret_cast<void ***>(v), 1, ops); }
+void h_VecView(void *x, OPS *ops) { h_MultiVecView(reinterpret_cast<void **>(x), 0, 1, ops); }
+void h_VecInnerProd(void *x, void *y, double *ip, OPS *ops)
+{
+    int s[2] = {0, 0}, e[2] = {1, 1};
+    h_MultiVecInnerProd('S', reinterpret_cast<void **>(x), reinterpret_cast<void **>(y), 0, s,
+                        e, ip, 1, ops);
+}
+void h_VecSetRandomValue(void *x, OPS *ops)
+{
+    h_MultiVecSetRandomValue(reinterpret_cast<void **>(x), 0, 1, ops);
+}
+void h_VecAxpby(double a, void *x, double b, void *y, OPS *ops)
+{
+    int s[2] = {0, 0}, e[2] = {1, 1};
+    h_MultiVecAxpby(a, reinterpret_cast<void **>(x), b, reinterpret_cast<void **>(y), s, e, ops);
+}
+void h_MatDotVec(void *m, void *x, void *y, OPS *ops)
+{
+    int s[2] = {0, 0}, e[2] = {1, 1};
+    h_MatDotMultiVec(m, reinterpret_cast<void **>(x), reinterpret_cast<void **>(y), s, e, ops);
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ C ABI
+extern "C" {
+
+void OPS_HIP_Set(OPS *ops)
+{
+    if (ops->lapack_ops != nullptr)
+        fail("OPS_HIP_Set: ops->lapack_ops must be NULL (as for OPS_CCS_Set)", __FILE__,
+             __LINE__);
+    (void)ctx();
+    OPS *lap = static_cast<OPS *>(calloc(1, sizeof(OPS)));
+    OPS_HOSTDENSE_Set(lap);
+    ops->lapack_ops = lap;
+    ops->Printf = gcge_host_Printf;
+    ops->GetWtime = gcge_host_GetWtime;
+    ops->GetOptionFromCommandLine = gcge_host_GetOptionFromCommandLine;
+    ops->MatView = h_MatView;
+    ops->MatAxpby = nullptr;
+    ops->VecCreateByMat = h_VecCreateByMat;
+    ops->VecCreateByVec = h_VecCreateByVec;
+    ops->VecDestroy = h_VecDestroy;
+    ops->VecView = h_VecView;
+    ops->VecInnerProd = h_VecInnerProd;
+    ops->VecLocalInnerProd = h_VecInnerProd;
+    ops->VecSetRandomValue = h_VecSetRandomValue;
+    ops->VecAxpby = h_VecAxpby;
+    ops->MatDotVec = h_MatDotVec;
+    ops->MatTransDotVec = h_MatDotVec;
+    ops->MultiVecCreateByMat = h_MultiVecCreateByMat;
+    ops->MultiVecCreateByVec = h_MultiVecCreateByVec;
+    ops->MultiVecCreateByMultiVec = h_MultiVecCreateByMultiVec;
+    ops->MultiVecDestroy = h_MultiVecDestroy;
+    ops->GetVecFromMultiVec = h_GetVecFromMultiVec;
+    ops->RestoreVecForMultiVec = h_RestoreVecForMultiVec;
+    ops->MultiVecView = h_MultiVecView;
+    ops->MultiVecLocalInnerProd = h_MultiVecInnerProd;
+    ops->MultiVecInnerProd = h_MultiVecInnerProd;
+    ops->MultiVecSetRandomValue = h_MultiVecSetRandomValue;
+    ops->MultiVecAxpby = h_MultiVecAxpby;
+    ops->MultiVecLinearComb = h_MultiVecLinearComb;
+    ops->MatDotMultiVec = h_MatDotMultiVec;
+    ops->MatTransDotMultiVec = h_MatDotMultiVec;
+    ops->MultiVecQtAP = h_MultiVecQtAP;
+    ops->DenseMatQtAP = lap->DenseMatQtAP;
+    ops->DenseMatOrth = lap->DenseMatOrth;
+}
+
+void *HIPCCS_Create(const gcge_ccs_mat *host, int flags)
+{
+    if (host == nullptr || host->nrows <= 0 || host->nrows != host->ncols ||
+        host->j_col == nullptr || host->i_row == nullptr || host->data == nullptr)
+        return nullptr;
+    const int64_t n = host->nrows;
+    const int *jc = host->j_col;
+    if (jc[0] != 0) return nullptr;
+    for (int64_t j = 0; j < n; ++j)
+        if (jc[j + 1] < jc[j]) return nullptr;
+    const int64_t nnz = jc[n];
+    for (int64_t e = 0; e < nnz; ++e)
+        if (host->i_row[e] < 0 || host->i_row[e] >= n) return nullptr;
+    Ctx &c = ctx();
+    Mat *M = new Mat();
+    M->n = n;
+    SellMat &S = M->sell;
+    S.nrows = n;
+    S.nslices = (n + 63) / 64;
+    S.nnz_true = nnz;
+    std::vector<int64_t> sp((size_t)S.nslices + 1, 0);
+    int maxw = 0;
+    for (int64_t s = 0; s < S.nslices; ++s) {
+        int w = 0;
+        for (int64_t r = s * 64; r < (s + 1) * 64 && r < n; ++r) {
+            const int len = jc[r + 1] - jc[r];
+            if (len > w) w = len;
+        }
+        if (w > maxw) maxw = w;
+        sp[s + 1] = sp[s] + (int64_t)w * 64;
+    }
+    S.max_width = maxw;
+    S.nnz = sp[S.nslices];
+    // Column r of a symmetric CCS matrix is row r: fill SELL by rows.
+    std::vector<int> colv((size_t)(S.nnz > 0 ? S.nnz : 1));
+    std::vector<double> valv((size_t)(S.nnz > 0 ? S.nnz : 1));
+    for (int64_t s = 0; s < S.nslices; ++s) {
+        const int w = (int)((sp[s + 1] - sp[s]) / 64);
+        for (int l = 0; l < 64; ++l) {
+            const int64_t r = s * 64 + l;
+            const int len = r < n ? jc[r + 1] - jc[r] : 0;
+            for (int j = 0; j < w; ++j) {
+                const int64_t e = sp[s] + (int64_t)j * 64 + l;
+                if (j < len) {
+                    colv[e] = host->i_row[jc[r] + j];
+                    valv[e] = host->data[jc[r] + j];
+                } else {
+                    colv[e] = (int)(r < n ? r : 0);
+                    valv[e] = 0.0;
+                }
+            }
+        }
+    }
+    HIPCHECK(hipMalloc(reinterpret_cast<void **>(&S.slice_ptr), sizeof(int64_t) * sp.size()));
+    HIPCHECK(hipMalloc(reinterpret_cast<void **>(&S.col), sizeof(int) * colv.size()));
+    HIPCHECK(hipMalloc(reinterpret_cast<void **>(&S.val), sizeof(double) * valv.size()));
+    HIPCHECK(hipMemcpy(S.slice_ptr, sp.data(), sizeof(int64_t) * sp.size(), hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy(S.col, colv.data(), sizeof(int) * colv.size(), hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy(S.val, valv.data(), sizeof(double) * valv.size(), hipMemcpyHostToDevice));
+    M->bytes = (int64_t)(sizeof(int64_t) * sp.size() + sizeof(int) * colv.size() +
+                         sizeof(double) * valv.size());
+    if (flags & GCGE_HIP_MAT_FP32) {
+        std::vector<float> v32(valv.size());
+        for (size_t i = 0; i < valv.size(); ++i) v32[i] = (float)valv[i];
+        HIPCHECK(hipMalloc(reinterpret_cast<void **>(&S.val32), sizeof(float) * v32.size()));
+        HIPCHECK(hipMemcpy(S.val32, v32.data(), sizeof(float) * v32.size(), hipMemcpyHostToDevice));
+        M->bytes += (int64_t)(sizeof(float) * v32.size());
+    }
+    (void)c;
+    return M;
+}
+
+void HIPCCS_Destroy(void **mat)
+{
+    if (mat == nullptr || *mat == nullptr) return;
+    Mat *M = mat_of(*mat, "HIPCCS_Destroy");
+    ctx().sync();
+    HIPCHECK(hipFree(M->sell.slice_ptr));
+    HIPCHECK(hipFree(M->sell.col));
+    HIPCHECK(hipFree(M->sell.val));
+    if (M->sell.val32) HIPCHECK(hipFree(M->sell.val32));
+    M->magic = 0;
+    delete M;
+    *mat = nullptr;
+}
+
+void HIPCCS_Info(void *mat, int64_t *nrows, int64_t *nnz, int64_t *padded, int64_t *bytes)
+{
+    Mat *M = mat_of(mat, "HIPCCS_Info");
+    if (nrows) *nrows = M->n;
+    if (nnz) *nnz = M->sell.nnz_true;
+    if (padded) *padded = M->sell.nnz;
+    if (bytes) *bytes = M->bytes;
+}
+
+void HIP_MultiVecSetFromHost(void **mv, int start, int end, const double *src, int64_t ld)
+{
+    MV *m = mv_of(mv, "HIP_MultiVecSetFromHost");
+    check_range(m, start, end, "HIP_MultiVecSetFromHost");
+    Ctx &c = ctx();
+    for (int k = 0; k < end - start; ++k)
+        HIPCHECK(hipMemcpyAsync(m->d + (int64_t)(start + k) * m->ldd, src + (int64_t)k * ld,
+                                sizeof(double) * m->nrows, hipMemcpyHostToDevice, c.stream));
+    c.sync();
+}
+
+void HIP_MultiVecGetToHost(void **mv, int start, int end, double *dst, int64_t ld)
+{
+    MV *m = mv_of(mv, "HIP_MultiVecGetToHost");
+    check_range(m, start, end, "HIP_MultiVecGetToHost");
+    Ctx &c = ctx();
+    for (int k = 0; k < end - start; ++k)
+        HIPCHECK(hipMemcpyAsync(dst + (int64_t)k * ld, m->d + (int64_t)(start + k) * m->ldd,
+                                sizeof(double) * m->nrows, hipMemcpyDeviceToHost, c.stream));
+    c.sync();
+}
+
+void HIP_MultiVecInfo(void **mv, int64_t *nrows, int64_t *ncols, int64_t *ldd)
+{
+    MV *m = mv_of(mv, "HIP_MultiVecInfo");
+    if (nrows) *nrows = m->nrows;
+    if (ncols) *ncols = m->ncols;
+    if (ldd) *ldd = m->ldd;
+}
+
+void gcge_hip_set_random_mode(int mode, uint64_t seed)
+{
+    Ctx &c = ctx();
+    c.rand_mode = mode;
+    c.rand_seed = seed;
+    c.rand_counter = 0;
+}
+
+void gcge_hip_synchronize(void) { ctx().sync(); }
+
+int gcge_hip_last_error(char *buf, int len)
+{
+    std::lock_guard<std::mutex> lk(g_err_mu);
+    if (buf && len > 0) {
+        strncpy(buf, g_err.c_str(), (size_t)len - 1);
+        buf[len - 1] = 0;
+    }
+    return g_err_code;
+}
+
+int gcge_hip_device(void) { return ctx().device; }
+
+void gcge_hip_profile_enable(int on)
+{
+    Ctx &c = ctx();
+    c.prof.collect();
+    c.prof.on = on != 0;
+}
+
+void gcge_hip_profile_reset(void) { ctx().prof.reset(); }
+
+void gcge_hip_profile_read(int kind, int64_t *launches, double *total_ms, double *bytes,
+                           double *flops)
+{
+    Ctx &c = ctx();
+    c.prof.collect();
+    if (kind < 0 || kind >= PROF_NKIND) fail("gcge_hip_profile_read: bad kind", __FILE__, __LINE__);
+    if (launches) *launches = c.prof.launches[kind];
+    if (total_ms) *total_ms = c.prof.ms[kind];
+    if (bytes) *bytes = c.prof.bytes[kind];
+    if (flops) *flops = c.prof.flops[kind];
+}
+
+}  // extern "C"

@@ -1,0 +1,90 @@
+// Internal types of the HIP backend shared by the app, the driver and the
+// mixed-precision solver.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <vector>
+
+#include "kernels.h"
+
+namespace gcge {
+
+[[noreturn]] void fail(const char *what, const char *file, int line);
+void hip_check(hipError_t e, const char *expr, const char *file, int line);
+#define HIPCHECK(x) ::gcge::hip_check((x), #x, __FILE__, __LINE__)
+
+struct Stats {
+    int64_t spmm_calls = 0, spmm_cols = 0, gram_calls = 0, lincomb_calls = 0, dot_calls = 0;
+    double gram_flops = 0.0, lincomb_flops = 0.0;
+};
+
+// Per-kernel-class timing with HIP events recorded on the library stream
+// around each launch; elapsed times are collected when read.
+enum ProfKind { PROF_SPMM64 = 0, PROF_SPMM32 = 1, PROF_GRAM = 2, PROF_LINCOMB = 3,
+                PROF_BLAS1 = 4, PROF_NKIND = 5 };
+struct Profiler {
+    bool on = false;
+    struct Rec {
+        hipEvent_t a, b;
+        int kind;
+        double bytes, flops;
+    };
+    std::vector<Rec> pending;
+    std::vector<hipEvent_t> pool;
+    int64_t launches[PROF_NKIND] = {0};
+    double ms[PROF_NKIND] = {0}, bytes[PROF_NKIND] = {0}, flops[PROF_NKIND] = {0};
+    // returns the event to record after the launch, or null when off
+    hipEvent_t begin(int kind, double bytes, double flops, hipStream_t s);
+    void end(hipEvent_t e, hipStream_t s);
+    void collect();
+    void reset();
+};
+
+// One device, one stream per process; every slot enqueues on `stream` and
+// only slots that return host data synchronize.
+struct Ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    double *d_scratch = nullptr;
+    int64_t scratch_cap = 0;
+    double *d_small = nullptr;
+    int64_t small_cap = 0;
+    double *h_pinned = nullptr;
+    int64_t pinned_cap = 0;
+    int rand_mode = 0;
+    uint64_t rand_seed = 0;
+    uint64_t rand_counter = 0;
+    Stats stats;
+    Profiler prof;
+    double *scratch(int64_t ndouble);
+    double *small(int64_t ndouble);
+    double *pinned(int64_t ndouble);
+    void sync();
+};
+Ctx &ctx();
+
+// Device multivector: column-major nrows x ncols, leading dimension ldd.
+struct MV {
+    static constexpr uint32_t kMagic = 0x484d5631u;  // "HMV1"
+    uint32_t magic = kMagic;
+    int64_t nrows = 0, ncols = 0, ldd = 0;
+    double *d = nullptr;
+    bool owner = false;
+};
+
+// Device sparse matrix (SELL-64 of a symmetric CCS input).
+struct Mat {
+    static constexpr uint32_t kMagic = 0x484d5431u;  // "HMT1"
+    uint32_t magic = kMagic;
+    int64_t n = 0;
+    int64_t bytes = 0;
+    SellMat sell;
+};
+
+MV *mv_of(void **p, const char *where);
+Mat *mat_of(void *p, const char *where);
+MV *mv_create(int64_t nrows, int64_t ncols);
+void mv_destroy(MV *m);
+
+}  // namespace gcge

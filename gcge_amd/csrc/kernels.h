@@ -1,0 +1,70 @@
+// Launchers of the gfx950 kernels.  All take device pointers and run on
+// the given stream; none synchronizes.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace gcge {
+
+// Sparse matrix in SELL-64: rows grouped in slices of 64 (one wavefront),
+// slice s has width w_s = its longest row; entry j of lane l of slice s is
+// at slice_ptr[s] + j*64 + l.  Padding entries have value 0 and the row's
+// own index as column.
+struct SellMat {
+    int64_t nrows = 0;
+    int64_t nslices = 0;
+    int64_t nnz = 0;      // stored, including padding
+    int64_t nnz_true = 0; // of the CCS input
+    int max_width = 0;
+    int64_t *slice_ptr = nullptr;  // nslices + 1
+    int *col = nullptr;
+    double *val = nullptr;
+    float *val32 = nullptr;
+};
+
+// Y[:, c] = A X[:, c], c < ncols (FP64 / FP32).
+void spmm(const SellMat &A, const double *X, int64_t ldx, double *Y, int64_t ldy,
+          int ncols, hipStream_t s);
+void spmm32(const SellMat &A, const float *X, int64_t ldx, float *Y, int64_t ldy,
+            int ncols, hipStream_t s);
+// Y = A X + sigma * B X with A and B sharing the slice structure (FP32).
+void spmm_shift32(const SellMat &A, const SellMat &B, float sigma, const float *X,
+                  int64_t ldx, float *Y, int64_t ldy, int ncols, hipStream_t s);
+
+// Y = alpha X + beta Y over an nrows x ncols block; X null scales only;
+// beta == 0 overwrites Y without reading it.
+void axpby(int64_t nrows, int ncols, double alpha, const double *X, int64_t ldx,
+           double beta, double *Y, int64_t ldy, hipStream_t s);
+// Y[:, c] *= beta[c] (device array) -- or beta_scalar when beta == null.
+void scale_cols(int64_t nrows, int ncols, const double *beta, double beta_scalar,
+                double *Y, int64_t ldy, hipStream_t s);
+// Y[:, c] = X[:, c] (nrows x ncols).
+void copy_cols(int64_t nrows, int ncols, const double *X, int64_t ldx, double *Y,
+               int64_t ldy, hipStream_t s);
+// out[c] = X[:, c] . Y[:, c]; partial is scratch of dots_scratch(ncols) doubles.
+int64_t dots_scratch(int ncols);
+void col_dots(int64_t nrows, int ncols, const double *X, int64_t ldx, const double *Y,
+              int64_t ldy, double *out, double *partial, hipStream_t s);
+// G (m x n, ldg) = X^T Y with X: nrows x m, Y: nrows x n.  lower_only
+// computes only entries with row >= col (the 'S' case).  partial is scratch
+// of gram_scratch(m, n) doubles.
+int64_t gram_scratch(int64_t nrows, int m, int n);
+void gram(int64_t nrows, int m, int n, const double *X, int64_t ldx, const double *Y,
+          int64_t ldy, double *G, int ldg, bool lower_only, double *partial,
+          hipStream_t s);
+// Y (nrows x n) = X (nrows x m) C (m x n, ldc, device) + Y diag(beta)
+// beta_mode: 0 overwrite, 1 scalar beta_scalar, 2 per-column beta (device).
+void lincomb(int64_t nrows, int m, int n, const double *X, int64_t ldx, const double *C,
+             int ldc, int beta_mode, double beta_scalar, const double *beta, double *Y,
+             int64_t ldy, hipStream_t s);
+// U[0,1) from SplitMix64(seed, element index) into an nrows x ncols block
+// whose first element has global index offset.
+void random_uniform(int64_t nrows, int ncols, double *Y, int64_t ldy, uint64_t seed,
+                    uint64_t offset, hipStream_t s);
+// FP64 <-> FP32 conversions of an nrows x ncols block.
+void to_f32(int64_t nrows, int ncols, const double *X, int64_t ldx, float *Y,
+            int64_t ldy, hipStream_t s);
+void to_f64(int64_t nrows, int ncols, const float *X, int64_t ldx, double *Y,
+            int64_t ldy, hipStream_t s);
+
+}  // namespace gcge

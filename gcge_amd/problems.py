@@ -1,0 +1,142 @@
+"""Synthetic generalized eigenproblems A x = lambda B x, stored CCS.
+
+The constant-coefficient stencil problems have closed-form spectra; they
+are the analytic oracles of the eigensolver.  All matrices are symmetric,
+so their CCS arrays are also their CSR arrays (the HIP backend reads them
+as a row gather).
+
+Index arrays are int32 and values float64, the CCSMAT layout of
+app/app_ccs.h:20-24.
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass
+
+import numpy as np
+import scipy.sparse as sp
+
+
+@dataclass
+class CCSProblem:
+    """A pair of CCS matrices plus solver settings for one problem."""
+    name: str
+    A: sp.csc_matrix
+    B: sp.csc_matrix | None
+    nev: int
+    block_size: int
+    nev_init: int
+    nev_max: int
+    tol_abs: float = 1e-1
+    tol_rel: float = 1e-8
+    gap_min: float = 1e-5
+
+    @property
+    def n(self) -> int:
+        return self.A.shape[0]
+
+
+def _path_adjacency(n: int) -> sp.csr_matrix:
+    e = np.ones(n - 1)
+    return sp.diags([e, e], [-1, 1], shape=(n, n), format="csr")
+
+
+def grid_adjacency(n: int, dim: int) -> sp.csr_matrix:
+    """Adjacency of the n^dim interior grid with Dirichlet boundary."""
+    T = _path_adjacency(n)
+    I = sp.identity(n, format="csr")
+    if dim == 2:
+        return (sp.kron(I, T) + sp.kron(T, I)).tocsr()
+    if dim == 3:
+        II = sp.identity(n * n, format="csr")
+        return (sp.kron(II, T) + sp.kron(I, sp.kron(T, I)) + sp.kron(T, II)).tocsr()
+    raise ValueError("dim must be 2 or 3")
+
+
+def _canonical_csc(M: sp.spmatrix) -> sp.csc_matrix:
+    M = sp.csc_matrix(M)
+    M.sum_duplicates()
+    M.sort_indices()
+    M.indptr = M.indptr.astype(np.int32)
+    M.indices = M.indices.astype(np.int32)
+    M.data = M.data.astype(np.float64)
+    return M
+
+
+def stencil_pair(n: int, dim: int):
+    """A = 2d I - Adj, B = (4d I + Adj) / (6d) ... for d = 2: (8I+Adj)/12,
+    for d = 3: (12I+Adj)/18."""
+    Adj = grid_adjacency(n, dim)
+    N = n ** dim
+    I = sp.identity(N, format="csr")
+    if dim == 2:
+        A = 4.0 * I - Adj
+        B = (8.0 * I + Adj) / 12.0
+    else:
+        A = 6.0 * I - Adj
+        B = (12.0 * I + Adj) / 18.0
+    return _canonical_csc(A), _canonical_csc(B)
+
+
+def analytic_eigenvalues(n: int, dim: int, count: int) -> np.ndarray:
+    """Smallest `count` eigenvalues of stencil_pair(n, dim), ascending.
+
+    With mu = sum_k 2 cos(i_k pi / (n+1)) the eigenvalues of Adj, A and B
+    share eigenvectors, so lambda = (2d - mu) * s / (s' + mu) with
+    (s, s') = (12, 8) in 2D and (18, 12) in 3D.
+    """
+    c = 2.0 * np.cos(np.arange(1, n + 1) * math.pi / (n + 1))
+    if dim == 2:
+        mu = (c[:, None] + c[None, :]).ravel()
+        lam = (4.0 - mu) * 12.0 / (8.0 + mu)
+    else:
+        mu = (c[:, None, None] + c[None, :, None] + c[None, None, :]).ravel()
+        lam = (6.0 - mu) * 18.0 / (12.0 + mu)
+    lam.sort()
+    return lam[:count]
+
+
+def slepc_block_policy(nev: int):
+    """Block size / nevInit / nevMax chosen as the SLEPc plugin does
+    (gcge.c:394-431)."""
+    if nev > 6:
+        if nev < 20:
+            bs = nev // 3
+        elif nev < 1000:
+            bs = nev // 5
+        elif nev < 4000:
+            bs = 100
+        else:
+            bs = 200
+        nev_init = 3 * bs
+        nev_max = nev_init + nev
+    else:
+        bs = nev
+        nev_init = 2 * nev
+        nev_max = nev_init
+    return bs, nev_init, nev_max
+
+
+def stencil_problem(n: int, dim: int, nev: int, block_size: int | None = None,
+                    nev_init: int | None = None, nev_max: int | None = None,
+                    name: str | None = None) -> CCSProblem:
+    A, B = stencil_pair(n, dim)
+    if block_size is None:
+        block_size, nev_init, nev_max = slepc_block_policy(nev)
+    return CCSProblem(name or f"stencil{dim}d_n{n}", A, B, nev, block_size,
+                      nev_init, nev_max)
+
+
+def config1() -> CCSProblem:
+    """2D 128x128 interior, nev = 20, SLEPc block policy (bs 4, 12, 32)."""
+    return stencil_problem(128, 2, 20, name="cfg1_2d_128")
+
+
+def config2(block_size: int = 120) -> CCSProblem:
+    """3D 128^3 interior, nev = 100, block size 120 with nevInit = nevMax
+    = 220 (the largest nevMax the driver's workspace asserts allow at
+    nevInit < 3 bs, src/ops_eig_sol_gcg.c:1280-1284)."""
+    if block_size == 120:
+        return stencil_problem(128, 3, 100, 120, 220, 220, name="cfg2_3d_128")
+    bs, ni, nm = slepc_block_policy(100)
+    return stencil_problem(128, 3, 100, bs, ni, nm, name="cfg2_3d_128_slepc")

@@ -1,0 +1,184 @@
+/*
+ * gcge_hip.h -- C ABI of the MI355X (gfx950) backend of the GCG eigensolver.
+ *
+ * Plain pointers and sizes only; no HIP or torch types cross this
+ * boundary.  Multivector and matrix handles are opaque; `coef`, `beta`,
+ * `inner_prod` and `qAp` arguments of the table slots are host arrays,
+ * exactly as in the reference table (src/ops.h:43-152).
+ *
+ * Errors: the slots return void, as in the reference, which aborts on a
+ * failed assert.  This backend prints the failing call and aborts the
+ * same way; gcge_hip_last_error() additionally reports the last recorded
+ * failure for callers that install their own abort handler.
+ */
+#ifndef GCGE_HIP_H
+#define GCGE_HIP_H
+
+#include <stdint.h>
+#include "gcge_ops.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------ */
+/* The drop-in: a HIP "app" that fills the operation table.            */
+/* ------------------------------------------------------------------ */
+
+/* Fills every multivector, vector and matrix slot of `ops` with HIP
+ * implementations, and installs the host small-dense table as
+ * ops->lapack_ops.  Replaces OPS_CCS_Set (app/app_ccs.c:213-249): like
+ * it, it expects ops->lapack_ops == NULL and is followed by OPS_Setup.
+ * MatAxpby stays NULL, so the driver never shifts A in place
+ * (src/ops_eig_sol_gcg.c:600-630).  MultiVecQtAP is filled with a HIP
+ * implementation of the semantics of DefaultMultiVecQtAP
+ * (src/ops_multi_vec.c:351-411), including op(A)*P left in mv_ws. */
+void OPS_HIP_Set(OPS *ops);
+
+/* Host small-dense table over the LAPACKVEC layout (gcge_dense_mv);
+ * replaces OPS_LAPACK_Set (app/app_lapack.c:970-1014). */
+void OPS_HOSTDENSE_Set(OPS *ops);
+
+/* DenseMatQtAP and DenseMatOrth of the host table (app/app_lapack.c:
+ * 24-227 and 653-699). */
+void gcge_host_DenseMatQtAP(char ntluA, char nsdC, int nrowsA, int ncolsA,
+                            int nrowsC, int ncolsC, double alpha, double *matQ,
+                            int ldQ, double *matA, int ldA, double *matP, int ldP,
+                            double beta, double *matC, int ldC, double *dbl_ws);
+void gcge_host_DenseMatOrth(double *mat, int nrows, int ldm, int start, int *end,
+                            double orth_zero_tol, double *dbl_ws, int length,
+                            int *int_ws);
+void gcge_host_Printf(const char *fmt, ...);
+/* Sends the table's Printf output (the driver's monitor lines) to stderr
+ * instead of stdout when to_stderr != 0. */
+void gcge_host_set_print_target(int to_stderr);
+double gcge_host_GetWtime(void);
+int gcge_host_GetOptionFromCommandLine(const char *name, char type, void *value,
+                                       int argc, char *argv[], OPS *ops);
+
+/* ------------------------------------------------------------------ */
+/* Device objects                                                      */
+/* ------------------------------------------------------------------ */
+
+/* Uploads a CCS matrix (CCSMAT layout, app/app_ccs.h:20-24) to the
+ * current device and returns the handle the table slots take as `mat`.
+ * The matrix must be symmetric, as the generalized symmetric-definite
+ * problem requires: the device kernels read the compressed columns as
+ * compressed rows (the reading the reference's MKL path makes,
+ * app/app_ccs.c:78-82).  A single-precision copy of the values is kept
+ * when flags & GCGE_HIP_MAT_FP32.  Returns NULL on invalid input. */
+#define GCGE_HIP_MAT_FP32 1
+void *HIPCCS_Create(const gcge_ccs_mat *host, int flags);
+void HIPCCS_Destroy(void **mat);
+/* Number of rows, stored nonzeros (including SELL padding) and device
+ * bytes of a matrix handle. */
+void HIPCCS_Info(void *mat, int64_t *nrows, int64_t *nnz, int64_t *padded,
+                 int64_t *bytes);
+
+/* Host <-> device copies of columns [start, end) of a multivector; the
+ * host array is column-major with leading dimension ld. */
+void HIP_MultiVecSetFromHost(void **mv, int start, int end, const double *src,
+                             int64_t ld);
+void HIP_MultiVecGetToHost(void **mv, int start, int end, double *dst, int64_t ld);
+/* Rows, columns and leading dimension of a multivector handle. */
+void HIP_MultiVecInfo(void **mv, int64_t *nrows, int64_t *ncols, int64_t *ldd);
+
+/* Random fill of MultiVecSetRandomValue:
+ *   GCGE_HIP_RAND_LIBC   (default) host libc rand()/(RAND_MAX+1), column by
+ *                        column, uploaded -- the reference's sequence
+ *                        (app/app_lapack.c:322-333), so a run started
+ *                        after srand(s) sees the same initial block;
+ *   GCGE_HIP_RAND_DEVICE U[0,1) from a counter-based generator on the
+ *                        device (SplitMix64 of seed and element index). */
+#define GCGE_HIP_RAND_LIBC 0
+#define GCGE_HIP_RAND_DEVICE 1
+void gcge_hip_set_random_mode(int mode, uint64_t seed);
+
+/* Waits for all work queued by this library. */
+void gcge_hip_synchronize(void);
+/* Copies the last error message (empty if none) into buf; returns its
+ * code (0 = none). */
+int gcge_hip_last_error(char *buf, int len);
+/* Number of the visible device the library uses. */
+int gcge_hip_device(void);
+
+/* Kernel timing: when enabled, HIP events are recorded on the library's
+ * stream around each launch of the kernel classes below, and read back
+ * with their algorithmic bytes and flops (bytes: every operand read once
+ * and every result written once).
+ *   kind 0: FP64 SpMM (MatDotMultiVec), 1: FP32 shifted SpMM of the
+ *   device BlockPCG, 2: Gram (inner products, both kernels),
+ *   3: LinearComb, 4: BLAS-1. */
+void gcge_hip_profile_enable(int on);
+void gcge_hip_profile_reset(void);
+void gcge_hip_profile_read(int kind, int64_t *launches, double *total_ms, double *bytes,
+                           double *flops);
+
+/* ------------------------------------------------------------------ */
+/* The GCG driver                                                      */
+/* ------------------------------------------------------------------ */
+/* A reentrant restatement of the reference driver
+ * (src/ops_eig_sol_gcg.c:64-1562) that calls only through the table; it
+ * runs over OPS_HIP_Set (the FP64 GPU path) or OPS_HOSTDENSE_Set.  The
+ * names mirror the reference entry points with the HIPGCG_ prefix so both
+ * libraries can be linked into one program. */
+
+/* EigenSolverSetup_GCG (src/ops_eig_sol_gcg.c:1565-1628): stores the
+ * sizes, tolerances and workspaces and sets ops->EigenSolver.  mv_ws[0]
+ * has nevMax + 2 block_size columns, mv_ws[1..3] block_size columns
+ * (as created by HIPGCG_Create). */
+void HIPGCG_EigenSolverSetup(double gapMin, int nevInit, int nevMax, int block_size,
+                             double tol[2], int numIterMax, int print, int printtime,
+                             int user_defined_multi_linear_solver, void **mv_ws[4],
+                             double *dbl_ws, int *int_ws, OPS *ops);
+/* GCGE_Setparameters (src/ops_eig_sol_gcg.c:1878-1918). */
+void HIPGCG_Setparameters(double gapMin, int shift, OPS *ops);
+/* EigenSolverSetParameters_GCG (src/ops_eig_sol_gcg.c:1684-1741). */
+void HIPGCG_EigenSolverSetParameters(
+    int check_conv_max_num, const char *initX_orth_method, int initX_orth_block_size,
+    int initX_orth_max_reorth, double initX_orth_zero_tol, const char *compP_orth_method,
+    int compP_orth_block_size, int compP_orth_max_reorth, double compP_orth_zero_tol,
+    const char *compW_orth_method, int compW_orth_block_size, int compW_orth_max_reorth,
+    double compW_orth_zero_tol, int compW_cg_max_iter, double compW_cg_rate,
+    double compW_cg_tol, const char *compW_cg_tol_type, int compW_cg_auto_shift,
+    int compRR_min_num, double compRR_min_gap, double compRR_tol, OPS *ops);
+/* GCGE_Create / GCGE_Destroymvws (src/ops_eig_sol_gcg.c:1865-1877,
+ * 1919-1927); dbl_ws and int_ws are sized and allocated here. */
+void HIPGCG_Create(void *A, int nevMax, int block_size, int nevInit, void ***mv_ws,
+                   double **dbl_ws, int **int_ws, OPS *ops);
+void HIPGCG_Destroy(void ***mv_ws, double *dbl_ws, int *int_ws, int nevMax,
+                    int block_size, OPS *ops);
+/* Releases the solver state stored in ops->eigen_solver_workspace. */
+void HIPGCG_Free(OPS *ops);
+/* Results of the last solve: stats[0] numIter, stats[1] nevConv,
+ * stats[2] total BlockPCG iterations, stats[3] linear solves;
+ * times[0..9] phase seconds (initX, checkconv, compP, compRR, rr_matW,
+ * dsyevx, compRV, compW, compX, linsol; the names of
+ * src/ops_eig_sol_gcg.c:27-41). */
+void HIPGCG_GetStats(OPS *ops, int64_t *stats, double *times);
+
+/* Dense symmetric eigensolver used by the Rayleigh-Ritz step:
+ * GCGE_EIG_HOST (own Householder + implicit QL on the host) or
+ * GCGE_EIG_DEVICE (rocSOLVER dsyevd on the device). */
+#define GCGE_EIG_HOST 0
+#define GCGE_EIG_DEVICE 1
+void HIPGCG_SetDenseEigensolver(OPS *ops, int which);
+
+/* ------------------------------------------------------------------ */
+/* Mixed precision: FP32 BlockPCG in the MultiLinearSolver slot        */
+/* ------------------------------------------------------------------ */
+/* Installs a device BlockPCG that runs in single precision as
+ * ops->MultiLinearSolver (the user-defined solver of
+ * src/ops_eig_sol_gcg.c:585-642, user_defined_multi_linear_solver = 1).
+ * A and B must carry FP32 copies (GCGE_HIP_MAT_FP32).  Same stopping
+ * rule as BlockPCG (src/ops_lin_sol.c:383-394): per column, residual
+ * below rate * initial or below tol, at most max_iter iterations.
+ * sigma is the fixed shift of A + sigma B (0 when autoshift is off). */
+void HIPGCG_MultiLinearSolverSetup_BlockPCG32(int max_iter, double rate, double tol,
+                                              void *B, double sigma, OPS *ops);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GCGE_HIP_H */

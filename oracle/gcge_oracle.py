@@ -1,0 +1,644 @@
+"""CPU oracle for the GCG eigensolver hot path (numpy/scipy restatement).
+
+TEST INFRASTRUCTURE.  Only tests/, __graft_entry__.smoke() and bench.py's
+cpu_baseline leg may import this module, and only as the checker.  The
+product (gcge_amd) never calls it.
+
+It restates, in floating-point numpy, the reference algorithm of
+xie-group/gcge for A x = lambda B x:
+
+* the operator-table slot semantics the HIP backend must reproduce
+  (app/app_ccs.c:50-139, app/app_lapack.c:24-534, src/ops_multi_vec.c:
+  202-230, 351-411);
+* BlockPCG (src/ops_lin_sol.c:140-437);
+* block modified Gram-Schmidt in the B inner product
+  (src/ops_orth.c:45-118, 203-394);
+* the GCG driver with soft locking, P/W construction and Rayleigh-Ritz
+  (src/ops_eig_sol_gcg.c:64-1562) and its parameter policy
+  (src/ops_eig_sol_gcg.c:1565-1628, 1878-1918).
+
+Multivectors are (N, k) numpy arrays; a column range [s, e) of the
+reference is a slice [:, s:e].  The small dense eigensolve (dsyevx in the
+reference, src/ops_eig_sol_gcg.c:1207) is scipy.linalg.eigh.
+
+How each part is pinned (tests/test_oracle.py):
+* slots: dense numpy brute force and the compiled reference's own slots
+  called through its operator table (oracle/_ref);
+* BlockPCG: known answer on SPD systems (exact CG solution at max_iter >= N)
+  and the compiled reference's eigensolver trajectory;
+* MGS: Q^T B Q = I and span preservation on small inputs;
+* the driver: same iteration count, converged count and eigenvalues as the
+  compiled reference on the same seeded inputs, and the closed-form spectra
+  of the stencil problems.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+import scipy.linalg as sla
+
+DBL_EPSILON = np.finfo(np.float64).eps
+
+_libc = ctypes.CDLL(None)
+_libc.rand.restype = ctypes.c_int
+RAND_MAX = 2147483647
+
+
+def libc_srand(seed: int) -> None:
+    _libc.srand(ctypes.c_uint(seed))
+
+
+def libc_uniform(nrows: int, ncols: int) -> np.ndarray:
+    """U[0,1) from libc rand(), column by column (app/app_lapack.c:322-333)."""
+    out = np.empty((nrows, ncols), order="F")
+    flat = out.reshape(-1, order="F")
+    r = _libc.rand
+    for i in range(flat.size):
+        flat[i] = r()
+    flat /= (RAND_MAX + 1.0)
+    return out
+
+
+# --------------------------------------------------------------------------
+# Slot semantics
+# --------------------------------------------------------------------------
+
+def spmm(A, X: np.ndarray) -> np.ndarray:
+    """y = A x per column, A in CCS (app/app_ccs.c:116-131); A None is a
+    copy (app/app_ccs.c:134-137)."""
+    if A is None:
+        return np.array(X, copy=True)
+    return np.asarray(A @ X)
+
+
+def inner_prod(nsd: str, X: np.ndarray, Y: np.ndarray) -> np.ndarray:
+    """'N': X^T Y; 'S': lower triangle of X^T Y mirrored to the upper;
+    'D': column dots (app/app_lapack.c:66-181 via :299-313)."""
+    if nsd == "D":
+        return np.einsum("ij,ij->j", X, Y)
+    G = X.T @ Y
+    if nsd == "S":
+        assert G.shape[0] == G.shape[1]
+        L = np.tril(G)
+        G = L + np.tril(G, -1).T
+    return G
+
+
+def store_block(ip: np.ndarray, G: np.ndarray, nsd: str, ld_rows: int | None = None):
+    """Write G into the host array `ip` as the reference does with ldIP:
+    full block for 'N'/'S', the diagonal with stride ldIP for 'D'."""
+    if nsd == "D":
+        ip[: G.size] = G
+    else:
+        ip[: G.shape[0], : G.shape[1]] = G
+
+
+def axpby(alpha: float, X: np.ndarray | None, beta: float, Y: np.ndarray) -> None:
+    """Y <- alpha X + beta Y in place; beta == 0 overwrites (the memset of
+    app/app_lapack.c:350-351), X None scales only."""
+    if beta == 0.0:
+        Y[...] = 0.0
+    elif beta != 1.0:
+        Y *= beta
+    if X is not None:
+        Y += alpha * X
+
+
+def linear_comb(X: np.ndarray | None, Y: np.ndarray, coef: np.ndarray | None,
+                beta) -> None:
+    """Y <- X coef + Y diag(beta) in place (app/app_lapack.c:463-534).
+
+    beta None overwrites with X coef (and leaves Y alone when X or coef is
+    None: the reference's gamma = 0 only enters its dgemm); a scalar beta
+    scales all columns; an array scales column k by beta[k]."""
+    if beta is None:
+        if X is None or coef is None:
+            return
+        Y[...] = 0.0
+    else:
+        b = np.asarray(beta, dtype=np.float64)
+        if b.ndim == 0:
+            if float(b) != 1.0:
+                Y *= float(b)
+        else:
+            Y *= b[None, : Y.shape[1]]
+    if X is not None and coef is not None:
+        Y += X @ coef
+
+
+def qtap(ntsA: str, ntsdQAP: str, Q: np.ndarray, A, P: np.ndarray,
+         ws: np.ndarray | None) -> np.ndarray:
+    """Q^T op(A) P (src/ops_multi_vec.c:351-411).
+
+    With A given, ws[:, :P.shape[1]] receives A P as a side effect that the
+    orthogonalization reuses (src/ops_orth.c:316-323).  'T' returns the
+    transpose of the product."""
+    if A is None:
+        AP = P
+    else:
+        AP = spmm(A, P)
+        if ws is not None:
+            ws[:, : P.shape[1]] = AP
+    nsd = ntsdQAP
+    if nsd == "T":
+        return inner_prod("N", AP, Q)
+    return inner_prod(nsd, Q, AP)
+
+
+# --------------------------------------------------------------------------
+# Parameters
+# --------------------------------------------------------------------------
+
+@dataclass
+class GCGParams:
+    """Effective parameters after EigenSolverSetup_GCG + GCGE_Setparameters
+    (src/ops_eig_sol_gcg.c:1573-1626, 1878-1918)."""
+    nev_init: int
+    nev_max: int
+    block_size: int
+    tol: tuple = (1e-1, 1e-8)
+    gap_min: float = 1e-5
+    num_iter_max: int = 1000
+    shift: int = 1                       # compW_cg_auto_shift
+    check_conv_max_num: int = 50
+    orth_block_size: int = -1
+    orth_max_reorth: int = 2
+    orth_zero_tol: float = 2 * DBL_EPSILON
+    reorth_tol: float = 50 * DBL_EPSILON  # src/ops_orth.c:404
+    cg_max_iter: int = 30
+    cg_rate: float = 1e-2
+    cg_tol: float = 1e-14
+    cg_shift: float = 0.0
+    user_linear_solver: object = None     # callable(A, B, sigma, b, x) or None
+
+
+# --------------------------------------------------------------------------
+# BlockPCG
+# --------------------------------------------------------------------------
+
+def block_pcg(apply_op, b: np.ndarray, x: np.ndarray, max_iter: int, rate: float,
+              tol: float, ws_r: np.ndarray, ws_p: np.ndarray, ws_w: np.ndarray,
+              trace: list | None = None) -> int:
+    """Per-column CG on op (src/ops_lin_sol.c:140-437, tol_type "abs").
+
+    `apply_op(src, dst, zws)` writes op(src) into dst using zws as scratch,
+    as MatDotMultiVecShift does (src/ops_eig_sol_gcg.c:64-97).  Columns stop
+    once their residual drops below rate * initial or below tol; stopped
+    columns are skipped, and the remaining ones are grouped into contiguous
+    blocks for the operator application (src/ops_lin_sol.c:256-311).
+    b is clobbered as scratch after the initial residual, as in the
+    reference (it is the z workspace of the shifted operator).
+    Returns the number of iterations.
+    """
+    nu = b.shape[1]
+    r = ws_r[:, :nu]
+    p = ws_p[:, :nu]
+    w = ws_w[:, :nu]
+    norm_b = np.ones(nu)
+    apply_op(x, r, p)
+    axpby(1.0, b, -1.0, r)
+    rho2_all = inner_prod("D", r, r)
+    init_res = np.sqrt(rho2_all)
+    unconv = [i for i in range(nu) if init_res[i] > tol * norm_b[i]]
+    rho2 = np.array([rho2_all[i] for i in unconv])
+    rho1 = np.zeros_like(rho2)
+    last_res = np.zeros(nu)
+    niter = 0
+    while niter < max_iter and len(unconv) > 0:
+        blocks = [0]
+        for i in range(1, len(unconv)):
+            if unconv[i] - unconv[i - 1] > 1:
+                blocks.append(i)
+        blocks.append(len(unconv))
+        pTw = np.zeros(len(unconv))
+        for bi in range(len(blocks) - 1):
+            for col in range(blocks[bi], blocks[bi + 1]):
+                beta = 0.0 if niter == 0 else rho2[col] / rho1[col]
+                c = unconv[col]
+                axpby(1.0, r[:, c], beta, p[:, c])
+            s = unconv[blocks[bi]]
+            e = unconv[blocks[bi + 1] - 1] + 1
+            apply_op(p[:, s:e], w[:, s:e], b[:, 0:e - s])
+            pTw[blocks[bi]:blocks[bi + 1]] = inner_prod("D", p[:, s:e], w[:, s:e])
+        rho1 = rho2.copy()
+        for bi in range(len(blocks) - 1):
+            for col in range(blocks[bi], blocks[bi + 1]):
+                alpha = rho2[col] / pTw[col]
+                c = unconv[col]
+                axpby(alpha, p[:, c], 1.0, x[:, c])
+                axpby(-alpha, w[:, c], 1.0, r[:, c])
+            s = unconv[blocks[bi]]
+            e = unconv[blocks[bi + 1] - 1] + 1
+            rho2[blocks[bi]:blocks[bi + 1]] = inner_prod("D", r[:, s:e], r[:, s:e])
+        for i, c in enumerate(unconv):
+            last_res[c] = math.sqrt(rho2[i])
+        keep = [i for i, c in enumerate(unconv)
+                if last_res[c] > rate * init_res[c] and last_res[c] > tol * norm_b[c]]
+        unconv = [unconv[i] for i in keep]
+        rho1 = rho1[keep]
+        rho2 = rho2[keep]
+        niter += 1
+    if trace is not None:
+        trace.append(("bpcg", nu, niter))
+    return niter
+
+
+# --------------------------------------------------------------------------
+# Modified Gram-Schmidt in the B inner product
+# --------------------------------------------------------------------------
+
+def _orth_self(V: np.ndarray, s: int, e: int, B, max_reorth: int, zero_tol: float,
+               reorth_tol: float, ws) -> int:
+    """Column-by-column B-orthonormalization of V[:, s:e] in place
+    (src/ops_orth.c:45-118).  A column whose B-norm is below zero_tol is
+    replaced by the last column and the range shrinks.  Returns the new end."""
+    if e <= s:
+        return e
+    k = s
+    while k < e:
+        col = V[:, k:k + 1]
+        r = qtap("S", "N", V[:, k:e], B, col, ws)[:, 0]
+        rk = math.sqrt(r[0])
+        if rk < zero_tol:
+            if k < e - 1:
+                V[:, k] = V[:, e - 1]
+            e -= 1
+            continue
+        inv = 1.0 / rk
+        V[:, k] *= inv
+        if k < e - 1:
+            coef = r[1:] * (-inv)
+            V[:, k + 1:e] += np.outer(V[:, k], coef)
+            for _ in range(1, max_reorth - 1):
+                c2 = -qtap("S", "N", V[:, k + 1:e], B, V[:, k:k + 1], ws)[:, 0]
+                V[:, k + 1:e] += np.outer(V[:, k], c2)
+                if np.max(np.abs(c2)) < reorth_tol:
+                    break
+        k += 1
+    return e
+
+
+def mgs_orth(V: np.ndarray, start_x: int, end_x: int, B, block_size: int,
+             max_reorth: int, zero_tol: float, reorth_tol: float,
+             ws: np.ndarray | None) -> int:
+    """Block MGS of V[:, start_x:end_x] against V[:, :start_x] and within
+    itself, in the B inner product (src/ops_orth.c:203-394).  Returns the
+    new end (dependent columns are dropped).  `ws` receives B times the
+    current block, which the re-orthogonalization passes reuse
+    (src/ops_orth.c:316-323)."""
+    if end_x <= start_x:
+        return end_x
+    if ws is None:
+        ws = np.zeros((V.shape[0], end_x))
+    if start_x > 0:
+        for _ in range(1 + max_reorth):
+            coef = qtap("S", "N", V[:, :start_x], B, V[:, start_x:end_x], ws)
+            coef = -coef
+            V[:, start_x:end_x] += V[:, :start_x] @ coef
+            if np.max(np.abs(coef)) < reorth_tol:
+                break
+    init_start = start_x
+    if block_size <= 0:
+        block_size = max((end_x - init_start) // 2, 2)
+    block_size = min(block_size, end_x - init_start)
+    while block_size > 0:
+        s1 = init_start
+        e1 = _orth_self(V, s1, s1 + block_size, B, max_reorth, zero_tol,
+                        reorth_tol, ws)
+        init_end = e1
+        length = block_size - (e1 - s1)
+        length = min(length, end_x - e1 - length)
+        if length > 0:
+            V[:, init_end:init_end + length] = V[:, end_x - length:end_x]
+        end_x = end_x - (block_size - (init_end - init_start))
+        if init_end < end_x and init_start < init_end:
+            nb = init_end - init_start
+            for it in range(1 + max_reorth):
+                if B is not None and it > 0:
+                    coef = qtap("S", "T", V[:, init_end:end_x], None, ws[:, :nb], None)
+                else:
+                    coef = qtap("S", "T", V[:, init_end:end_x], B,
+                                V[:, init_start:init_end], ws)
+                coef = -coef
+                V[:, init_end:end_x] += V[:, init_start:init_end] @ coef
+                if np.max(np.abs(coef)) < reorth_tol:
+                    break
+        init_start = init_end
+        block_size = min(block_size, end_x - init_start)
+    return end_x
+
+
+# --------------------------------------------------------------------------
+# GCG driver
+# --------------------------------------------------------------------------
+
+@dataclass
+class GCGResult:
+    eval: np.ndarray
+    evec: np.ndarray
+    nev_conv: int
+    num_iter: int
+    trace: list = field(default_factory=list)
+
+
+def gcg_solve(A, B, nev: int, prm: GCGParams, X0: np.ndarray | None = None,
+              trace: bool = False, verbose: bool = False) -> GCGResult:
+    """GCG for the `nev` smallest pairs of A x = lambda B x
+    (src/ops_eig_sol_gcg.c:1259-1562, with InitializeX :102-159,
+    CheckConvergence :196-321, ComputeP :322-463, ComputeX :464-477,
+    ComputeW :478-701, ComputeRayleighRitz :931-1258, ComputeRitzVec
+    :160-195).
+
+    X0: the initial block (N x nevInit); None draws it from libc rand()
+    after srand(0), as the SLEPc plugin's sequence does (gcge.c:508).
+    """
+    N = A.shape[0]
+    bs, nev_init, nev_max = prm.block_size, prm.nev_init, prm.nev_max
+    assert nev_init <= nev_max
+    assert nev_init >= 3 * bs or nev_init == nev_max
+    assert nev_max >= nev + bs and nev_max <= nev + nev_init
+    tr: list = [] if trace else None
+
+    V = np.zeros((N, nev_max + 2 * bs), order="F")
+    ritz = np.zeros((N, nev_max), order="F")
+    ws = [np.zeros((N, bs), order="F") for _ in range(3)]
+    ss_eval = np.ones(nev_max + 2 * bs)
+
+    # sizes (src/ops_eig_sol_gcg.c:1291-1297)
+    st = dict(sizeC=0, sizeN=bs, sizeX=nev_init, sizeP=0, sizeW=0)
+    st["sizeV"] = st["sizeX"]
+    st["startN"] = 0
+    st["endN"] = bs
+    st["endX"] = st["sizeX"]
+    st["startP"] = st["endX"]
+    st["endP"] = st["startP"]
+    st["startW"] = st["endP"]
+    st["endW"] = st["startW"]
+
+    def orth(Vm, s, e, Bm, wsm):
+        return mgs_orth(Vm, s, e, Bm, prm.orth_block_size, prm.orth_max_reorth,
+                        prm.orth_zero_tol, prm.reorth_tol, wsm)
+
+    # InitializeX (:102-159), nevGiven = 0
+    if X0 is None:
+        libc_srand(0)
+        X0 = libc_uniform(N, st["sizeX"])
+    V[:, : st["sizeX"]] = X0
+    endX = orth(V, 0, st["sizeX"], B, ritz)
+    assert endX == st["sizeX"]
+
+    mats = {"ss_matA": None, "ss_evec": None}
+
+    def rayleigh_ritz(nev_conv: int):
+        """(:931-1258)"""
+        sC, sX, sP, sW = st["sizeC"], st["sizeX"], st["sizeP"], st["sizeW"]
+        PtAP = None
+        if sP > 0:
+            M = mats["ss_matA"]
+            Pc = mats["ss_evec"][:, sX - sC: sX - sC + sP]
+            PtAP = Pc.T @ (M @ Pc)
+            PtAP = np.tril(PtAP) + np.tril(PtAP, -1).T
+        st["sizeV"] = sX + sP + sW
+        st["startN"] += nev_conv - sC
+        st["endN"] += nev_conv - sC
+        st["endN"] = min(st["endN"], st["endX"])
+        st["sizeN"] = st["endN"] - st["startN"]
+        st["sizeC"] = nev_conv
+        sC = st["sizeC"]
+        nV = st["sizeV"] - sC
+        M = np.zeros((nV, nV))
+        if sW > 0:
+            s0, eW, sWs = st["startN"], st["endW"], st["startW"]
+            blk = qtap("S", "N", V[:, s0:eW], A, V[:, sWs:eW], ws[0])
+            M[:, sX + sP - sC:] = blk
+            M[sX + sP - sC:, : sX + sP - sC] = blk[: sX + sP - sC, :].T
+        if sX == st["sizeV"]:
+            length = sX - sC
+            b = min(bs, length)
+            c0 = sC
+            while length:
+                M[:, c0 - sC: c0 - sC + b] = qtap("S", "N", V[:, sC:sX], A,
+                                                  V[:, c0:c0 + b], ws[0])
+                length -= b
+                c0 += b
+                b = min(b, length)
+        else:
+            M[: sX + sP - sC, : sX + sP - sC] = 0.0
+            np.fill_diagonal(M[: sX - sC, : sX - sC], ss_eval[sC:sX])
+            if sP > 0:
+                M[sX - sC: sX - sC + sP, sX - sC: sX - sC + sP] = PtAP
+        # the reference reads the upper triangle (UPLO='U'); symmetrize from it
+        Mu = np.triu(M) + np.triu(M, 1).T
+        w, Z = sla.eigh(Mu + prm.cg_shift * np.eye(nV) if prm.cg_shift else Mu)
+        if prm.cg_shift:
+            w = w - prm.cg_shift
+        ss_eval[sC: sC + nV] = w
+        mats["ss_matA"] = Mu
+        mats["ss_evec"] = Z
+        if tr is not None:
+            tr.append(("rr", nV, sC))
+
+    def ritz_vec():
+        """(:160-195)"""
+        sN, eW, eX = st["startN"], st["endW"], st["endX"]
+        Z = mats["ss_evec"]
+        ritz[:, sN:eX] = V[:, sN:eW] @ Z[: eW - sN, : eX - sN]
+
+    def check_convergence(num_check: int):
+        """(:196-321); returns (nevConv, offsets)."""
+        sN, sC = st["startN"], st["sizeC"]
+        X = ritz[:, sN:sN + num_check]
+        R = spmm(A, X) - spmm(B, X) * ss_eval[sN:sN + num_check][None, :]
+        res = np.sqrt(inner_prod("D", R, R))
+        lam = ss_eval[sN:sN + num_check]
+        unconv = (res > prm.tol[0]) | (res > np.abs(lam) * prm.tol[1])
+        idx = 0
+        while idx < num_check and not unconv[idx]:
+            idx += 1
+        if verbose and idx < num_check:
+            # the monitor line of src/ops_eig_sol_gcg.c:241-245
+            print("%3d EPS nconv=%d first unconverged value (error(abs,rel)) "
+                  "%6.5e (%6.5e, %6.5e)" % (counter[0] + 1, sN, lam[idx], res[idx],
+                                             res[idx] / abs(lam[idx])))
+        while idx > 0:
+            a = ss_eval[sN + idx - 1]
+            if abs((a - ss_eval[sN + idx]) / a) > prm.gap_min:
+                break
+            idx -= 1
+        nev_conv = sC + idx
+        offs = []
+        state, nu = True, 0
+        cur = None
+        for i in range(num_check):
+            if unconv[i]:
+                if state:
+                    cur = sN + i
+                    state = False
+                nu += 1
+                if nu == st["sizeN"]:
+                    offs.append((cur, sN + i + 1))
+                    break
+            else:
+                if not state:
+                    offs.append((cur, sN + i))
+                    state = True
+        if nu < st["sizeN"]:
+            if state:
+                cur = sN + num_check
+            e = min(sN + num_check + st["sizeN"] - nu, st["endX"])
+            assert cur < e
+            offs.append((cur, e))
+        if tr is not None:
+            tr.append(("check", num_check, nev_conv, tuple(offs)))
+        return nev_conv, offs
+
+    def compute_p(offs):
+        """(:322-463)"""
+        sC, sX = st["sizeC"], st["sizeX"]
+        nV = st["sizeV"] - sC
+        Z = mats["ss_evec"]
+        bsz = 0
+        for (o1, o2) in offs:
+            L = o2 - o1
+            Z[:, sX - sC + bsz: sX - sC + bsz + L] = Z[:, o1 - sC: o2 - sC]
+            bsz += L
+        sP = bsz
+        for (o1, o2) in offs:
+            Z[o1 - sC: o2 - sC, sX - sC: sX - sC + sP] = 0.0
+        startP = sX - sC
+        endP = orth(Z[:, : startP + sP], startP, startP + sP, None,
+                    np.zeros((nV, sP)))
+        st["startP"] = startP + sC
+        st["endP"] = endP + sC
+        st["sizeP"] = st["endP"] - st["startP"]
+        sP = st["sizeP"]
+        coef = Z[:, sX - sC: sX - sC + sP]
+        tmp = V[:, st["startN"]: st["endW"]] @ coef[: st["endW"] - st["startN"], :]
+        V[:, st["startP"]: st["endP"]] = tmp
+
+    def compute_w(offs):
+        """(:478-701)"""
+        sC = st["sizeC"]
+        sigma = 0.0
+        if prm.shift == 1:
+            sigma = -ss_eval[sC] + (ss_eval[sC + 1] - ss_eval[sC]) * 0.01
+        sigma = prm.cg_shift + sigma
+        st["startW"] = st["endP"]
+        bsz = 0
+        b0 = offs[0][0]
+        for (o1, o2) in offs:
+            L = o2 - o1
+            V[:, st["startW"] + bsz: st["startW"] + bsz + L] = ritz[:, o1:o2]
+            ritz[:, b0 + bsz: b0 + bsz + L] = spmm(B, V[:, o1:o2]) * \
+                (ss_eval[o1:o2] + sigma)[None, :]
+            bsz += L
+        st["endW"] = st["startW"] + bsz
+        bcols = ritz[:, b0: b0 + bsz]
+        xcols = V[:, st["startW"]: st["endW"]]
+
+        def apply_op(src, dst, zws):
+            dst[...] = spmm(A, src)
+            if sigma != 0.0:
+                if B is None:
+                    dst += sigma * src
+                else:
+                    zws[...] = spmm(B, src)
+                    dst += sigma * zws
+
+        if prm.user_linear_solver is not None:
+            prm.user_linear_solver(A, B, sigma, bcols, xcols)
+        else:
+            block_pcg(apply_op, bcols, xcols, prm.cg_max_iter, prm.cg_rate,
+                      prm.cg_tol, ws[0], ws[1], ws[2], tr)
+        endW = orth(V, st["startW"], st["endW"], B, ws[0])
+        st["endW"] = endW
+        st["sizeW"] = st["endW"] - st["startW"]
+
+    counter = [0]
+    rayleigh_ritz(0)
+    ss_eval[st["sizeV"]:] = ss_eval[st["sizeV"] - 1]
+    ritz_vec()
+
+    nev0 = min(nev, nev_max)
+    nev_conv = 0
+    nev_target = 2 * bs if nev_init < nev_max else nev0
+    nev_target = min(nev_target, nev0)
+    num_iter_max = prm.num_iter_max
+    num_iter = 0
+    offs_w = None
+    offs_p = None
+    while True:
+        if num_iter <= 0:
+            num_check = 0
+        else:
+            num_check = st["sizeN"] if st["startN"] + st["sizeN"] < st["endX"] \
+                else st["endX"] - st["startN"]
+        num_check = min(num_check, prm.check_conv_max_num)
+        if num_check == 0:
+            num_check = 1
+        nev_conv, offs_w = check_convergence(num_check)
+        if nev_conv >= nev_target:
+            if nev_conv >= nev0:
+                break
+            # grow X by P and W (:1410-1431)
+            nev_target = min(nev_target + st["sizeP"] + st["sizeW"], nev0)
+            sC = st["sizeC"]
+            old_endX = st["endX"]
+            st["sizeX"] = min(st["sizeX"] + st["sizeP"] + st["sizeW"], nev_max)
+            Z = mats["ss_evec"]
+            ritz[:, old_endX: st["sizeX"]] = V[:, st["startN"]: st["endW"]] @ \
+                Z[: st["endW"] - st["startN"], old_endX - sC: st["sizeX"] - sC]
+            st["sizeP"] = 0
+            st["sizeW"] = 0
+            st["sizeV"] = st["sizeX"]
+            st["startP"] = st["endX"]
+            st["endP"] = st["startP"]
+            st["startW"] = st["endP"]
+            st["endW"] = st["startW"]
+            st["endX"] = st["sizeX"]
+            st["endN"] = min(st["startN"] + bs, st["endX"])
+            st["sizeN"] = st["endN"] - st["startN"]
+            num_iter_max -= num_iter
+            num_iter = 0
+        if num_iter == 0:
+            st["sizeP"] = 0
+            st["startP"] = st["endX"]
+            st["endP"] = st["startP"]
+        else:
+            compute_p(offs_p)
+        # ComputeX (:464-477)
+        V[:, st["startN"]: st["endX"]] = ritz[:, st["startN"]: st["endX"]]
+        compute_w(offs_w)
+        offs_p = offs_w
+        rayleigh_ritz(nev_conv)
+        ss_eval[st["sizeV"]:] = ss_eval[st["sizeV"] - 1]
+        ritz_vec()
+        num_iter += 1
+        counter[0] += 1
+        if num_iter >= num_iter_max:
+            break
+    total_iter = num_iter + (prm.num_iter_max - num_iter_max)
+    sX = st["sizeX"]
+    return GCGResult(ss_eval[:sX].copy(), ritz[:, :sX].copy(), nev_conv,
+                     total_iter, tr or [])
+
+
+# --------------------------------------------------------------------------
+# Convenience
+# --------------------------------------------------------------------------
+
+def relative_residuals(A, B, lam: np.ndarray, X: np.ndarray) -> np.ndarray:
+    """||A x - lambda B x|| / |lambda| per column, the check of
+    src/ops_eig_sol_gcg.c:239-240 divided through by |lambda|."""
+    R = spmm(A, X) - spmm(B, X) * lam[None, :]
+    return np.sqrt(np.einsum("ij,ij->j", R, R)) / np.abs(lam)
+
+
+def dense_generalized_eigenvalues(A, B, count: int) -> np.ndarray:
+    """Smallest `count` eigenvalues by dense LAPACK (dsygv via scipy)."""
+    Ad = A.toarray() if hasattr(A, "toarray") else np.asarray(A)
+    Bd = B.toarray() if hasattr(B, "toarray") else np.asarray(B)
+    return sla.eigh(Ad, Bd, eigvals_only=True, subset_by_index=[0, count - 1])

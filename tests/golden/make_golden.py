@@ -1,0 +1,67 @@
+"""Writes tests/golden/ref_runs.json from the reference GCGE compiled in
+oracle/_ref (oracle/Makefile), on the seeded stencil problems of
+gcge_amd.problems.
+
+Each record: problem, sizes, shift, the reference's eigenvalues (first
+nevConv + 5), numIter and nevConv, for the CCS app (app/app_ccs.c) and,
+for small problems, the dense LAPACK app (app/app_lapack.c).
+
+Run in the build container (the reference sources and MKL are needed to
+build oracle/_ref):  python tests/golden/make_golden.py
+"""
+import json
+import os
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+
+import numpy as np  # noqa: E402
+
+from gcge_amd import problems as P  # noqa: E402
+from ref_binding import RefLib  # noqa: E402
+
+CASES = [
+    # name, n, dim, nev, shift, apps
+    ("cfg1_2d_128", 128, 2, 20, 1, ("ccs",)),
+    ("cfg1_2d_128", 128, 2, 20, 0, ("ccs",)),
+    ("2d_32", 32, 2, 10, 1, ("ccs", "lapack")),
+    ("2d_32", 32, 2, 10, 0, ("ccs", "lapack")),
+    ("2d_24", 24, 2, 20, 1, ("ccs", "lapack")),
+    ("3d_12", 12, 3, 30, 1, ("ccs",)),
+    ("3d_12", 12, 3, 30, 0, ("ccs",)),
+    ("3d_16", 16, 3, 40, 0, ("ccs",)),
+]
+
+
+def main():
+    ref = RefLib()
+    out = []
+    for name, n, dim, nev, shift, apps in CASES:
+        prob = P.stencil_problem(n, dim, nev)
+        for app in apps:
+            if app == "ccs":
+                r = ref.solve_ccs(prob, shift=shift)
+            else:
+                ops = ref.lapack_table()
+                hA, ka = ref.wrap_dense(prob.A.toarray())
+                hB, kb = ref.wrap_dense(prob.B.toarray())
+                r = ref.solve(ops, hA, hB, prob.n, prob.nev, prob.block_size, prob.nev_init,
+                              prob.nev_max, (prob.tol_abs, prob.tol_rel), prob.gap_min, shift)
+            k = min(r["nev_conv"] + 5, prob.nev_max)
+            rec = {"name": name, "n": n, "dim": dim, "nev": nev, "shift": shift, "app": app,
+                   "block_size": prob.block_size, "nev_init": prob.nev_init,
+                   "nev_max": prob.nev_max, "num_iter": r["num_iter"],
+                   "nev_conv": r["nev_conv"], "eval": [float(x) for x in r["eval"][:k]]}
+            print(name, app, shift, r["num_iter"], r["nev_conv"], flush=True)
+            out.append(rec)
+    with open(os.path.join(HERE, "ref_runs.json"), "w") as f:
+        json.dump({"source": "reference GCGE compiled from its sources by oracle/Makefile, "
+                             "MKL 2021.4 BLAS/LAPACK, one process, no MPI",
+                   "runs": out}, f, indent=1)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,104 @@
+"""End-to-end parity of the GPU eigensolver.
+
+* This library's driver over the HIP table (the FP64 path) and with the
+  FP32 device BlockPCG (the mixed path), against the closed-form spectra,
+  the reference's recorded runs (tests/golden/ref_runs.json) and the
+  residual bound the driver promises (src/ops_eig_sol_gcg.c:239-240).
+* The reference driver itself (oracle/_ref) running unchanged over the HIP
+  table: the drop-in.
+"""
+import ctypes as C
+import json
+import os
+
+import numpy as np
+import pytest
+
+import gcge_oracle as O
+from gcge_amd import abi, problems as P
+from gcge_amd import solver as S
+
+pytestmark = pytest.mark.gpu
+
+GOLD = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "ref_runs.json")))
+
+
+def golden(name, shift, app="ccs"):
+    for r in GOLD["runs"]:
+        if r["name"] == name and r["shift"] == shift and r["app"] == app:
+            return r
+    raise KeyError(name)
+
+
+def check_pairs(prob, res, dim, n, iter_ref=None, iter_slack=6, nconv_ref=None):
+    k = res.nev_conv
+    assert k >= prob.nev
+    lam = res.eval[:k]
+    exact = P.analytic_eigenvalues(n, dim, k)
+    rel = np.abs(lam - exact) / exact
+    assert rel.max() <= 1e-10, rel.max()
+    if res.evec is not None:
+        rr = O.relative_residuals(prob.A, prob.B, lam, res.evec[:, :k])
+        assert rr.max() <= 1e-8 * 1.01, rr.max()
+    if iter_ref is not None:
+        assert abs(res.num_iter - iter_ref) <= iter_slack, (res.num_iter, iter_ref)
+    if nconv_ref is not None:
+        assert res.nev_conv == nconv_ref
+
+
+@pytest.mark.parametrize("shift", [1, 0])
+def test_fp64_cfg1(hip, shift):
+    prob = P.config1()
+    g = golden("cfg1_2d_128", shift)
+    res = S.solve_problem(prob, shift=shift, want_vectors=True)
+    check_pairs(prob, res, 2, 128, g["num_iter"], 6, g["nev_conv"])
+    ref = np.array(g["eval"][: res.nev_conv])
+    assert np.max(np.abs(res.eval[: res.nev_conv] - ref) / ref) <= 1e-9
+
+
+def test_fp64_3d_small(hip):
+    prob = P.stencil_problem(12, 3, 30)
+    g = golden("3d_12", 1)
+    res = S.solve_problem(prob, shift=1, want_vectors=True)
+    check_pairs(prob, res, 3, 12, g["num_iter"], 8, g["nev_conv"])
+
+
+def test_mixed_cfg1(hip):
+    prob = P.config1()
+    res = S.solve_problem(prob, shift=0, mode="mixed", want_vectors=True)
+    check_pairs(prob, res, 2, 128)
+
+
+def test_mixed_3d(hip):
+    prob = P.stencil_problem(16, 3, 40)
+    res = S.solve_problem(prob, shift=0, mode="mixed", want_vectors=True)
+    check_pairs(prob, res, 3, 16)
+
+
+def test_device_random_and_eig(hip):
+    """Device RNG for the initial block and the device dense eigensolver:
+    a different trajectory, the same pairs."""
+    prob = P.stencil_problem(32, 2, 10)
+    res = S.solve_problem(prob, shift=1, rand="device", seed=3, eig="device",
+                          want_vectors=True)
+    check_pairs(prob, res, 2, 32)
+
+
+def test_dropin_reference_driver(hip, reflib):
+    """The reference GCG driver (compiled from its sources, oracle/_ref)
+    runs unchanged over the table filled by OPS_HIP_Set."""
+    prob = P.stencil_problem(32, 2, 10)
+    g = golden("2d_32", 1)
+    ops = reflib.empty_table()
+    hip.OPS_HIP_Set(C.cast(C.c_void_p(ops), abi.OPS_p))
+    reflib.setup(ops)
+    dA, dB = S.DeviceMatrix(prob.A), S.DeviceMatrix(prob.B)
+    hip.gcge_hip_set_random_mode(abi.GCGE_HIP_RAND_LIBC, 0)
+    r = reflib.solve(ops, dA.handle.value, dB.handle.value, prob.n, prob.nev, prob.block_size,
+                     prob.nev_init, prob.nev_max, (prob.tol_abs, prob.tol_rel), prob.gap_min,
+                     1, 1000, host_vectors=False)
+    k = r["nev_conv"]
+    assert k >= prob.nev
+    exact = P.analytic_eigenvalues(32, 2, k)
+    assert np.max(np.abs(r["eval"][:k] - exact) / exact) <= 1e-10
+    assert abs(r["num_iter"] - g["num_iter"]) <= 4
