@@ -627,6 +627,12 @@ void *HIPCCS_Create(const gcge_ccs_mat *host, int flags)
     }
     S.max_width = maxw;
     S.nnz = sp[S.nslices];
+    uint64_t h = 1469598103934665603ull;  // FNV-1a over the slice pointers
+    for (int64_t v : sp) {
+        h ^= (uint64_t)v;
+        h *= 1099511628211ull;
+    }
+    S.structure_hash = h;
     // Column r of a symmetric CCS matrix is row r: fill SELL by rows.
     std::vector<int> colv((size_t)(S.nnz > 0 ? S.nnz : 1));
     std::vector<double> valv((size_t)(S.nnz > 0 ? S.nnz : 1));

@@ -16,6 +16,7 @@ struct SellMat {
     int64_t nnz = 0;      // stored, including padding
     int64_t nnz_true = 0; // of the CCS input
     int max_width = 0;
+    uint64_t structure_hash = 0;   // of the slice widths, to match A and B
     int64_t *slice_ptr = nullptr;  // nslices + 1
     int *col = nullptr;
     double *val = nullptr;

@@ -5,9 +5,13 @@
 // It solves (A + sigma B) x_c = b_c for the columns of a block with the
 // stopping rule of BlockPCG (src/ops_lin_sol.c:383-394): a column stops
 // once its residual is below rate * initial residual or below tol, or
-// after max_iter iterations.  GCG tolerates the inexact inner solve: the
-// FP64 residual check and Rayleigh-Ritz step of the outer iteration act
-// as iterative refinement.
+// after max_iter iterations.
+//
+// One call is one step of mixed-precision iterative refinement: the
+// residual r = b - op(x) is formed in FP64 (k_resid64), the correction d
+// solves op(d) = r in FP32 from d = 0, and x += d in FP64 (k_add_up).  GCG
+// tolerates the inexact inner solve; its FP64 residual check and
+// Rayleigh-Ritz step refine further.
 //
 // Per iteration, for the active columns only (their indices are compacted
 // on the device):
@@ -69,21 +73,92 @@ __device__ __forceinline__ void block_sum_tile(float (&v)[kT], float *red /* [4]
     __syncthreads();
 }
 
-// x32[:, c] = (float) x64[:, c]
-__global__ __launch_bounds__(kWG) void k_down(int64_t n, const double *__restrict__ X,
-                                              int64_t ldx, float *__restrict__ Y, int64_t ldy)
+// Residual of the FP64 system, rounded to FP32: for the columns c < nc,
+// R32[:, c] = fl32(b[:, c] - (A + sigma B) x[:, c]) with the products in
+// FP64 from the FP64 matrix values, and partial[c * nblk + block] = the
+// block's sum of R32^2.  This is the refinement step: the FP32 solve then
+// finds the correction, whose size is that of the residual, instead of
+// the whole solution, whose FP32 rounding would swamp the correction.
+template <int W>
+__global__ __launch_bounds__(kWG) void k_resid64(
+    int64_t n, const int64_t *__restrict__ slice_ptr, const int *__restrict__ colA,
+    const double *__restrict__ valA, const int *__restrict__ colB,
+    const double *__restrict__ valB, double sigma, int nc, const double *__restrict__ Bv,
+    int64_t ldb, const double *__restrict__ Xv, int64_t ldx, float *__restrict__ R32,
+    int64_t ldr, float *__restrict__ partial)
 {
-    const int c = blockIdx.y;
-    const int64_t r = (int64_t)blockIdx.x * kWG + threadIdx.x;
-    if (r < n) Y[r + (int64_t)c * ldy] = (float)X[r + (int64_t)c * ldx];
+    __shared__ float red[(kWG / 64 + 1) * kT];
+    const int64_t nblk = gridDim.x;
+    const int64_t blk = xcd_swizzle(blockIdx.x, nblk);
+    const int64_t row = blk * kWG + threadIdx.x;
+    const bool live = row < n;
+    const bool shifted = (sigma != 0.0) && (valB != nullptr);
+    int ca[W], cb[W];
+    double va[W], vb[W];
+    int w = 0;
+    if (live) {
+        const int64_t s = row >> 6;
+        const int lane = (int)(row & 63);
+        const int64_t base = slice_ptr[s];
+        w = (int)((slice_ptr[s + 1] - base) >> 6);
+#pragma unroll
+        for (int j = 0; j < W; ++j) {
+            if (j < w) {
+                const int64_t e = base + (int64_t)j * 64 + lane;
+                ca[j] = colA[e];
+                va[j] = valA[e];
+                cb[j] = shifted ? colB[e] : ca[j];
+                vb[j] = shifted ? sigma * valB[e] : 0.0;
+            } else {
+                ca[j] = cb[j] = (int)row;
+                va[j] = vb[j] = 0.0;
+            }
+        }
+    }
+    for (int c0 = 0; c0 < nc; c0 += kT) {
+        float sq[kT];
+#pragma unroll
+        for (int t = 0; t < kT; ++t) sq[t] = 0.f;
+        if (live) {
+#pragma unroll
+            for (int t = 0; t < kT; ++t) {
+                if (c0 + t < nc) {
+                    const double *xc = Xv + (int64_t)(c0 + t) * ldx;
+                    double acc = 0.0;
+#pragma unroll
+                    for (int j = 0; j < W; ++j)
+                        if (j < w) acc += va[j] * xc[ca[j]] + vb[j] * xc[cb[j]];
+                    const float r = (float)(Bv[row + (int64_t)(c0 + t) * ldb] - acc);
+                    R32[row + (int64_t)(c0 + t) * ldr] = r;
+                    sq[t] = r * r;
+                }
+            }
+        }
+        block_sum_tile(sq, red);
+        if (threadIdx.x < kT && c0 + (int)threadIdx.x < nc)
+            partial[(int64_t)(c0 + threadIdx.x) * nblk + blockIdx.x] =
+                red[kWG / 64 * kT + threadIdx.x];
+        __syncthreads();
+    }
 }
 
-__global__ __launch_bounds__(kWG) void k_up(int64_t n, const float *__restrict__ X, int64_t ldx,
-                                            double *__restrict__ Y, int64_t ldy)
+// x[:, c] += (double) d[:, c]  and  d[:, c] = 0 (ready for the next solve)
+__global__ __launch_bounds__(kWG) void k_add_up(int64_t n, float *__restrict__ D, int64_t ldd,
+                                                double *__restrict__ Xv, int64_t ldx)
 {
     const int c = blockIdx.y;
     const int64_t r = (int64_t)blockIdx.x * kWG + threadIdx.x;
-    if (r < n) Y[r + (int64_t)c * ldy] = (double)X[r + (int64_t)c * ldx];
+    if (r < n) {
+        Xv[r + (int64_t)c * ldx] += (double)D[r + (int64_t)c * ldd];
+        D[r + (int64_t)c * ldd] = 0.f;
+    }
+}
+
+__global__ __launch_bounds__(kWG) void k_zero32(int64_t n, float *__restrict__ D, int64_t ldd)
+{
+    const int c = blockIdx.y;
+    const int64_t r = (int64_t)blockIdx.x * kWG + threadIdx.x;
+    if (r < n) D[r + (int64_t)c * ldd] = 0.f;
 }
 
 // Shifted SpMM on the listed columns with the p.w partial sums fused.
@@ -186,9 +261,17 @@ __global__ __launch_bounds__(kWG) void k_update_p(int64_t n, const int *__restri
     const float beta = first ? 0.f : rho[c] / rho_old[c];
     const float *r = R + (int64_t)c * ld;
     float *p = P + (int64_t)c * ld;
-    for (int64_t i = (int64_t)blockIdx.x * kWG + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * kWG)
-        p[i] = r[i] + beta * p[i];
+    // the first iteration overwrites p without reading it: the workspace
+    // may hold anything, including NaN
+    if (first) {
+        for (int64_t i = (int64_t)blockIdx.x * kWG + threadIdx.x; i < n;
+             i += (int64_t)gridDim.x * kWG)
+            p[i] = r[i];
+    } else {
+        for (int64_t i = (int64_t)blockIdx.x * kWG + threadIdx.x; i < n;
+             i += (int64_t)gridDim.x * kWG)
+            p[i] = r[i] + beta * p[i];
+    }
 }
 
 // x += alpha p, r -= alpha w on the active columns, with r.r partial sums.
@@ -222,32 +305,6 @@ __global__ __launch_bounds__(kWG) void k_update_xr(int64_t n, const int *__restr
 #pragma unroll
         for (int q = 0; q < kWG / 64; ++q) t += red[q];
         partial[(int64_t)k * gridDim.x + blockIdx.x] = t;
-    }
-}
-
-// r = b - r (initial residual, all columns), with r.r partial sums.
-__global__ __launch_bounds__(kWG) void k_resid0(int64_t n, const float *__restrict__ B,
-                                                float *__restrict__ R, int64_t ld,
-                                                float *__restrict__ partial)
-{
-    __shared__ float red[kWG / 64];
-    const int c = blockIdx.y;
-    const int64_t o = (int64_t)c * ld;
-    float s = 0.f;
-    for (int64_t i = (int64_t)blockIdx.x * kWG + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * kWG) {
-        const float rv = B[o + i] - R[o + i];
-        R[o + i] = rv;
-        s += rv * rv;
-    }
-    s = wave_sum(s);
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        float t = 0.f;
-#pragma unroll
-        for (int q = 0; q < kWG / 64; ++q) t += red[q];
-        partial[(int64_t)c * gridDim.x + blockIdx.x] = t;
     }
 }
 
@@ -324,7 +381,7 @@ struct PCG32 {
     // device workspace, FP32, ncap columns of n rows each
     int64_t n = 0;
     int ncap = 0;
-    float *b = nullptr, *x = nullptr, *r = nullptr, *p = nullptr, *w = nullptr;
+    float *x = nullptr, *r = nullptr, *p = nullptr, *w = nullptr;
     float *rho = nullptr, *rho_old = nullptr, *init_res = nullptr, *alpha = nullptr;
     float *partial = nullptr;
     int *cols = nullptr, *nact = nullptr;
@@ -337,7 +394,7 @@ struct PCG32 {
         n = nrows;
         ncap = nc;
         const size_t blkbytes = sizeof(float) * (size_t)n * nc;
-        for (float **q : {&b, &x, &r, &p, &w}) HIPCHECK(hipMalloc(reinterpret_cast<void **>(q), blkbytes));
+        for (float **q : {&x, &r, &p, &w}) HIPCHECK(hipMalloc(reinterpret_cast<void **>(q), blkbytes));
         for (float **q : {&rho, &rho_old, &init_res, &alpha})
             HIPCHECK(hipMalloc(reinterpret_cast<void **>(q), sizeof(float) * nc));
         HIPCHECK(hipMalloc(reinterpret_cast<void **>(&partial),
@@ -349,13 +406,13 @@ struct PCG32 {
     int64_t nparts_max() const { return (n + kWG - 1) / kWG; }
     void release()
     {
-        if (!b) return;
+        if (!x) return;
         ctx().sync();
-        for (float *q : {b, x, r, p, w, rho, rho_old, init_res, alpha, partial}) HIPCHECK(hipFree(q));
+        for (float *q : {x, r, p, w, rho, rho_old, init_res, alpha, partial}) HIPCHECK(hipFree(q));
         HIPCHECK(hipFree(cols));
         HIPCHECK(hipFree(nact));
         HIPCHECK(hipHostFree(h_nact));
-        b = nullptr;
+        x = nullptr;
     }
 };
 
@@ -401,26 +458,41 @@ void pcg32_solve(void *matA, void **mv_b, void **mv_x, int *start, int *end, OPS
         Bs = &Bm->sell;
         sigma = (float)S.sigma;
     }
+    // the shifted kernels walk A's slice pointers for both matrices
+    if (Bs != &A->sell && Bs->structure_hash != A->sell.structure_hash)
+        fail("BlockPCG32: a shift needs A and B with the same SELL row structure", __FILE__,
+             __LINE__);
     const int64_t n = A->n;
     S.ensure(n, nc);
     Ctx &c = ctx();
     hipStream_t st = c.stream;
     const dim3 gcol((unsigned)((n + kWG - 1) / kWG), nc);
-    k_down<<<gcol, kWG, 0, st>>>(n, Bv->d + (int64_t)start[0] * Bv->ldd, Bv->ldd, S.b, n);
-    k_down<<<gcol, kWG, 0, st>>>(n, Xv->d + (int64_t)start[1] * Xv->ldd, Xv->ldd, S.x, n);
-    // all columns listed for the initial residual r = b - op(x)
+    // refinement: r = fl32(b - op(x)) in FP64, solve op(d) = r from d = 0
+    k_zero32<<<gcol, kWG, 0, st>>>(n, S.x, n);
+    const unsigned nblk_row = (unsigned)((n + kWG - 1) / kWG);
+    {
+        const double *b64 = Bv->d + (int64_t)start[0] * Bv->ldd;
+        const double *x64 = Xv->d + (int64_t)start[1] * Xv->ldd;
+        const double *vB = (Bs != &A->sell) ? Bs->val : nullptr;
+        if (A->sell.max_width <= 8)
+            k_resid64<8><<<nblk_row, kWG, 0, st>>>(n, A->sell.slice_ptr, A->sell.col, A->sell.val,
+                                                   Bs->col, vB, S.sigma, nc, b64, Bv->ldd, x64,
+                                                   Xv->ldd, S.r, n, S.partial);
+        else if (A->sell.max_width <= 32)
+            k_resid64<32><<<nblk_row, kWG, 0, st>>>(n, A->sell.slice_ptr, A->sell.col,
+                                                    A->sell.val, Bs->col, vB, S.sigma, nc, b64,
+                                                    Bv->ldd, x64, Xv->ldd, S.r, n, S.partial);
+        else
+            fail("BlockPCG32: rows wider than 32 entries are not supported", __FILE__, __LINE__);
+    }
     {
         std::vector<int> all((size_t)nc);
         for (int k = 0; k < nc; ++k) all[k] = k;
         HIPCHECK(hipMemcpyAsync(S.cols, all.data(), sizeof(int) * nc, hipMemcpyHostToDevice, st));
-        *S.h_nact = nc;
-        HIPCHECK(hipMemcpyAsync(S.nact, S.h_nact, sizeof(int), hipMemcpyHostToDevice, st));
     }
-    spmm_pw_launch(A->sell, *Bs, sigma, S.cols, S.nact, S.x, S.r, n, S.partial, 0, st);
     const unsigned nb = blocks_for(n, 1024);
-    k_resid0<<<dim3(nb, nc), kWG, 0, st>>>(n, S.b, S.r, n, S.partial);
-    k_finish_rr<<<1, 1024, 0, st>>>(nc, 0, (int)nb, S.partial, S.cols, S.nact, S.rho, S.rho_old,
-                                    S.init_res, (float)S.rate, (float)S.tol);
+    k_finish_rr<<<1, 1024, 0, st>>>(nc, 0, (int)nblk_row, S.partial, S.cols, S.nact, S.rho,
+                                    S.rho_old, S.init_res, (float)S.rate, (float)S.tol);
     HIPCHECK(hipMemcpyAsync(S.h_nact, S.nact, sizeof(int), hipMemcpyDeviceToHost, st));
     c.sync();
     int nact = *S.h_nact;
@@ -448,7 +520,8 @@ void pcg32_solve(void *matA, void **mv_b, void **mv_x, int *start, int *end, OPS
         ++it;
     }
     S.iters += it;
-    k_up<<<gcol, kWG, 0, st>>>(n, S.x, n, Xv->d + (int64_t)start[1] * Xv->ldd, Xv->ldd);
+    // x += d in FP64 (and d back to zero)
+    k_add_up<<<gcol, kWG, 0, st>>>(n, S.x, n, Xv->d + (int64_t)start[1] * Xv->ldd, Xv->ldd);
 }
 
 }  // namespace

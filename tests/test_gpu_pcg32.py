@@ -1,0 +1,94 @@
+"""Known-answer tests of the single-precision device BlockPCG installed in
+the MultiLinearSolver slot (HIPGCG_MultiLinearSolverSetup_BlockPCG32).
+
+The stopping rule is the reference BlockPCG's (src/ops_lin_sol.c:383-394):
+per column, residual below rate * initial residual or below tol, at most
+max_iter iterations.  Residuals are measured in FP64 on the host.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+import scipy.sparse.linalg as spla
+
+from gcge_amd import abi, problems as P
+from gcge_amd.solver import DeviceMatrix, HipTable
+
+pytestmark = pytest.mark.gpu
+
+
+def ia(*v):
+    return (C.c_int * len(v))(*v)
+
+
+def run(table, lib, A, dA, dB, sigma, X0, Bm, max_iter, rate, ncols, b_off=3, x_off=5):
+    n = A.shape[0]
+    rng = np.random.default_rng(ncols)
+    b = rng.standard_normal((n, ncols + b_off))
+    x = np.zeros((n, ncols + x_off)) if X0 is None else X0
+    hb = table.create_multivec(b.shape[1], dA)
+    hx = table.create_multivec(x.shape[1], dA)
+    table.upload(hb, b)
+    table.upload(hx, x)
+    lib.HIPGCG_MultiLinearSolverSetup_BlockPCG32(max_iter, rate, 0.0,
+                                                 dB.handle if dB else None, sigma, table.ops)
+    abi.slot(table.ops, "MultiLinearSolver")(dA.handle, hb, hx, ia(b_off, x_off),
+                                             ia(b_off + ncols, x_off + ncols), table.ops)
+    xs = table.download(hx, 0, x.shape[1], n)
+    op = A + sigma * Bm if Bm is not None else A
+    bb = b[:, b_off:b_off + ncols]
+    xx = xs[:, x_off:x_off + ncols]
+    x0 = x[:, x_off:x_off + ncols]
+    r0 = np.linalg.norm(bb - op @ x0, axis=0)
+    r = np.linalg.norm(bb - op @ xx, axis=0)
+    # columns outside the solved range are untouched
+    assert np.array_equal(xs[:, :x_off], x[:, :x_off])
+    return r / r0, xx
+
+
+@pytest.fixture(scope="module")
+def setup(hip):
+    A, B = P.stencil_pair(40, 2)  # N = 1600
+    return HipTable(), hip, A, B, DeviceMatrix(A, fp32=True), DeviceMatrix(B, fp32=True)
+
+
+@pytest.mark.parametrize("ncols", [1, 5, 9, 33])
+def test_converges_to_fp32_accuracy(setup, ncols):
+    table, lib, A, B, dA, dB = setup
+    rel, xx = run(table, lib, A, dA, None, 0.0, None, None, 2000, 1e-6, ncols)
+    # FP32 recursion: the true residual trails the recursive one by about
+    # cond(A) * eps32 (cond ~ 1e3 here)
+    assert rel.max() <= 1e-4, rel.max()
+    # agrees with a direct FP64 solve to the conditioning of the problem
+    rng = np.random.default_rng(ncols)
+    b = rng.standard_normal((A.shape[0], ncols + 3))[:, 3:3 + ncols]
+    xe = spla.spsolve(A.tocsc(), b)
+    xe = xe.reshape(b.shape)
+    assert np.max(np.abs(xx - xe)) <= 1e-3 * np.max(np.abs(xe))
+
+
+def test_rate_stop(setup):
+    """With rate 1e-2 every column stops once its residual has dropped 100x
+    (and not far beyond: it stops at the first iteration below)."""
+    table, lib, A, B, dA, dB = setup
+    rel, _ = run(table, lib, A, dA, None, 0.0, None, None, 500, 1e-2, 12)
+    assert rel.max() <= 1e-2
+    assert rel.min() > 1e-5
+
+
+def test_max_iter_stop(setup):
+    table, lib, A, B, dA, dB = setup
+    rel1, _ = run(table, lib, A, dA, None, 0.0, None, None, 1, 1e-12, 4)
+    rel5, _ = run(table, lib, A, dA, None, 0.0, None, None, 5, 1e-12, 4)
+    assert np.all(rel1 < 1.0) and np.all(rel5 < rel1)
+
+
+def test_shifted_operator(setup):
+    """(A + sigma B) x = b with B's FP32 copy; nonzero initial guess."""
+    table, lib, A, B, dA, dB = setup
+    n = A.shape[0]
+    X0 = np.random.default_rng(9).standard_normal((n, 11))
+    rel, _ = run(table, lib, A, dA, dB, 0.37, X0, B, 2000, 1e-6, 6)
+    # FP32 recursion: the true residual trails the recursive one by about
+    # cond(A) * eps32 (cond ~ 1e3 here)
+    assert rel.max() <= 1e-4, rel.max()

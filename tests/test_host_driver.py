@@ -1,0 +1,113 @@
+"""This library's GCG driver, run on the host over its own dense table (no
+device use), against the reference driver's recorded runs over the
+reference dense app (tests/golden/ref_runs.json, app "lapack"), and the
+host small-dense operations (DenseMatQtAP, DenseMatOrth) against numpy.
+"""
+import ctypes as C
+import json
+import os
+
+import numpy as np
+import pytest
+
+import gcge_oracle as O
+from gcge_amd import abi, problems as P
+from gcge_amd import solver as S
+
+GOLD = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "ref_runs.json")))
+
+
+def golden(name, shift, app):
+    for r in GOLD["runs"]:
+        if r["name"] == name and r["shift"] == shift and r["app"] == app:
+            return r
+    raise KeyError((name, shift, app))
+
+
+@pytest.mark.parametrize("name,n,dim,nev,shift", [("2d_32", 32, 2, 10, 1),
+                                                  ("2d_32", 32, 2, 10, 0),
+                                                  ("2d_24", 24, 2, 20, 1)])
+def test_host_driver_matches_reference(hip, name, n, dim, nev, shift):
+    prob = P.stencil_problem(n, dim, nev)
+    g = golden(name, shift, "lapack")
+    r = S.solve_host_dense(prob.A.toarray(), prob.B.toarray(), nev, prob.block_size,
+                           prob.nev_init, prob.nev_max, shift=shift)
+    assert r.nev_conv == g["nev_conv"]
+    assert abs(r.num_iter - g["num_iter"]) <= 3
+    k = r.nev_conv
+    ref = np.array(g["eval"][:k])
+    assert np.max(np.abs(r.eval[:k] - ref) / ref) <= 1e-9
+    rr = O.relative_residuals(prob.A, prob.B, r.eval[:k], r.evec[:, :k])
+    assert rr.max() <= 1.01e-8
+
+
+def test_host_driver_max_iter_reports_nonconvergence(hip):
+    """numIterMax = 1: a reported non-convergence, not an error
+    (gcge.c:525-531 maps it to EPS_DIVERGED_ITS)."""
+    prob = P.stencil_problem(16, 2, 10)
+    r = S.solve_host_dense(prob.A.toarray(), prob.B.toarray(), 10, prob.block_size,
+                           prob.nev_init, prob.nev_max, shift=1, max_iter=1)
+    assert r.num_iter == 1 and r.nev_conv < 10
+
+
+@pytest.mark.parametrize("ntlu", ["L", "U", "N", "T"])
+@pytest.mark.parametrize("nsd", ["N", "S", "D"])
+def test_dense_qtap(hip, ntlu, nsd):
+    rng = np.random.default_rng(ord(ntlu) + ord(nsd))
+    n, m = 12, 5
+    A = rng.standard_normal((n, n))
+    if ntlu in "LU":
+        A = A + A.T
+        Ad = np.tril(A) if ntlu == "L" else np.triu(A)  # only that triangle is read
+        Ad = Ad + np.triu(rng.standard_normal((n, n)), 1) if ntlu == "L" else \
+            Ad + np.tril(rng.standard_normal((n, n)), -1)
+        opA = A
+    else:
+        Ad = A
+        opA = A if ntlu == "N" else A.T
+    Q = rng.standard_normal((n, m))
+    Pm = Q + 0.1 * rng.standard_normal((n, m)) if nsd == "S" else rng.standard_normal((n, m))
+    ld = m + 2
+    Cm = np.asfortranarray(rng.standard_normal((ld, m)))
+    C0 = Cm.copy()
+    Af, Qf, Pf = (np.asfortranarray(x) for x in (Ad, Q, Pm))
+    ws = np.zeros(n * m)
+    hip.gcge_host_DenseMatQtAP(ntlu.encode(), nsd.encode(), n, n, m, m, 0.5,
+                               Qf.ctypes.data_as(abi.c_dbl_p), n,
+                               Af.ctypes.data_as(abi.c_dbl_p), n,
+                               Pf.ctypes.data_as(abi.c_dbl_p), n, 2.0,
+                               Cm.ctypes.data_as(abi.c_dbl_p), ld,
+                               ws.ctypes.data_as(abi.c_dbl_p))
+    G = Q.T @ opA @ Pm
+    if nsd == "N":
+        want = 0.5 * G + 2.0 * C0[:m]
+        assert np.allclose(Cm[:m], want, atol=1e-12)
+    elif nsd == "S":
+        low = np.tril(0.5 * G + 2.0 * C0[:m])
+        want = low + np.tril(low, -1).T
+        assert np.allclose(Cm[:m], want, atol=1e-12)
+    else:
+        # 'D' writes dot c at matC[ldC*c] (app/app_lapack.c:75-113)
+        flat, flat0 = Cm.ravel(order="F"), C0.ravel(order="F")
+        want = 0.5 * np.diag(G) + 2.0 * flat0[0:ld * m:ld]
+        assert np.allclose(flat[0:ld * m:ld], want, atol=1e-12)
+
+
+def test_dense_orth(hip):
+    rng = np.random.default_rng(11)
+    m = 20
+    M = np.asfortranarray(rng.standard_normal((m, 9)))
+    Q0, _ = np.linalg.qr(M[:, :3])
+    M[:, :3] = Q0
+    M[:, 6] = M[:, 4] + M[:, 1]           # dependent on earlier columns
+    orig = M.copy()
+    end = C.c_int(9)
+    hip.gcge_host_DenseMatOrth(M.ctypes.data_as(abi.c_dbl_p), m, m, 3, C.byref(end), 1e-10,
+                               None, 0, None)
+    assert end.value == 8
+    Q = M[:, :end.value]
+    assert np.allclose(Q.T @ Q, np.eye(end.value), atol=1e-12)
+    # the new columns span the projected originals
+    proj = orig[:, 3:9] - Q0 @ (Q0.T @ orig[:, 3:9])
+    coef = np.linalg.lstsq(Q[:, 3:], proj, rcond=None)[0]
+    assert np.allclose(Q[:, 3:] @ coef, proj, atol=1e-10)
