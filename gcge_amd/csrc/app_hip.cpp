@@ -364,11 +364,14 @@ void h_MultiVecLinearComb(void **x, void **y, int, int *start, int *end, double 
     MV *X = mv_of(x, "MultiVecLinearComb x");
     check_range(X, start[0], end[0], "MultiVecLinearComb x");
     if (X->nrows != Y->nrows) fail("MultiVecLinearComb: row counts differ", __FILE__, __LINE__);
-    // stage coef (nr x nc, packed) and beta in one upload
+    // stage coef transposed and k-major (Ct[i * nc + k] = coef[i + k * ldc]),
+    // the operand layout of the matrix-core kernel, and beta, in one upload
     const int64_t ncoef = (int64_t)nr * nc;
     double *h = c.pinned(ncoef + nc);
-    for (int k = 0; k < nc; ++k)
-        memcpy(h + (int64_t)nr * k, coef + (int64_t)ldc * k, sizeof(double) * nr);
+    for (int k = 0; k < nc; ++k) {
+        const double *src = coef + (int64_t)ldc * k;
+        for (int i = 0; i < nr; ++i) h[(int64_t)i * nc + k] = src[i];
+    }
     int beta_mode = 0;
     double beta_scalar = 0.0;
     if (beta != nullptr) {
@@ -386,7 +389,7 @@ void h_MultiVecLinearComb(void **x, void **y, int, int *start, int *end, double 
     hipEvent_t ev = c.prof.begin(
         PROF_LINCOMB, 8.0 * (double)Y->nrows * (nr + nc * (beta_mode ? 2.0 : 1.0)),
         2.0 * (double)Y->nrows * nr * nc, c.stream);
-    lincomb(Y->nrows, nr, nc, X->d + (int64_t)start[0] * X->ldd, X->ldd, d, nr, beta_mode,
+    lincomb(Y->nrows, nr, nc, X->d + (int64_t)start[0] * X->ldd, X->ldd, d, nc, beta_mode,
             beta_scalar, d + ncoef, ys, Y->ldd, c.stream);
     c.prof.end(ev, c.stream);
     c.stats.lincomb_calls++;

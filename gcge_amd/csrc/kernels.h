@@ -53,11 +53,20 @@ int64_t gram_scratch(int64_t nrows, int m, int n);
 void gram(int64_t nrows, int m, int n, const double *X, int64_t ldx, const double *Y,
           int64_t ldy, double *G, int ldg, bool lower_only, double *partial,
           hipStream_t s);
-// Y (nrows x n) = X (nrows x m) C (m x n, ldc, device) + Y diag(beta)
+// Y (nrows x n) = X (nrows x m) C (m x n) + Y diag(beta), with C given
+// transposed and k-major: Ct[k * ldct + j] = C[k][j] (device).
 // beta_mode: 0 overwrite, 1 scalar beta_scalar, 2 per-column beta (device).
-void lincomb(int64_t nrows, int m, int n, const double *X, int64_t ldx, const double *C,
-             int ldc, int beta_mode, double beta_scalar, const double *beta, double *Y,
+void lincomb(int64_t nrows, int m, int n, const double *X, int64_t ldx, const double *Ct,
+             int ldct, int beta_mode, double beta_scalar, const double *beta, double *Y,
              int64_t ldy, hipStream_t s);
+// The matrix-core implementations behind gram() and lincomb().
+int64_t gram_mfma_scratch(int64_t nrows, int m, int n);
+void gram_mfma(int64_t nrows, int m, int n, const double *X, int64_t ldx, const double *Y,
+               int64_t ldy, double *G, int ldg, bool lower_only, double *partial,
+               hipStream_t s);
+void lincomb_mfma(int64_t nrows, int m, int n, const double *X, int64_t ldx, const double *Ct,
+                  int ldct, int beta_mode, double beta_scalar, const double *beta, double *Y,
+                  int64_t ldy, hipStream_t s);
 // U[0,1) from SplitMix64(seed, element index) into an nrows x ncols block
 // whose first element has global index offset.
 void random_uniform(int64_t nrows, int ncols, double *Y, int64_t ldy, uint64_t seed,

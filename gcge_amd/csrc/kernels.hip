@@ -407,72 +407,6 @@ __global__ __launch_bounds__(kWG) void k_gram_reduce(int m, int n, int tm, int t
 }
 
 // ---------------------------------------------------------------------
-// Y = X C + Y diag(beta): 64-row x 64-column output tiles, k staged
-// through LDS 32 at a time.
-// ---------------------------------------------------------------------
-__global__ __launch_bounds__(kWG) void k_lincomb(
-    int64_t nrows, int m, int n, const double *__restrict__ X, int64_t ldx,
-    const double *__restrict__ C, int ldc, int beta_mode, double beta_scalar,
-    const double *__restrict__ beta, double *__restrict__ Y, int64_t ldy, int ntn)
-{
-    __shared__ double sx[kGK][kGT + 1];  // [k][row]
-    __shared__ double sc[kGK][kGT + 1];  // [k][col]
-    const int64_t lb = xcd_swizzle(blockIdx.x, gridDim.x);
-    const int bj = (int)(lb % ntn);
-    const int64_t bi = lb / ntn;
-    const int64_t r0 = bi * kGT;
-    const int j0 = bj * kGT;
-    const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
-    double acc[4][4];
-#pragma unroll
-    for (int a = 0; a < 4; ++a)
-#pragma unroll
-        for (int b = 0; b < 4; ++b) acc[a][b] = 0.0;
-    for (int k0 = 0; k0 < m; k0 += kGK) {
-#pragma unroll
-        for (int q = 0; q < (kGK * kGT) / kWG; ++q) {
-            const int e = threadIdx.x + q * kWG;
-            const int rr = e % kGT, kk = e / kGT;
-            const int64_t r = r0 + rr;
-            sx[kk][rr] = (r < nrows && k0 + kk < m) ? X[r + (int64_t)(k0 + kk) * ldx] : 0.0;
-            const int cc = e % kGT, k2 = e / kGT;
-            sc[k2][cc] = (j0 + cc < n && k0 + k2 < m) ? C[(k0 + k2) + (int64_t)(j0 + cc) * ldc]
-                                                       : 0.0;
-        }
-        __syncthreads();
-#pragma unroll 8
-        for (int k = 0; k < kGK; ++k) {
-            double xa[4], cb[4];
-#pragma unroll
-            for (int a = 0; a < 4; ++a) xa[a] = sx[k][tx + 16 * a];
-#pragma unroll
-            for (int b = 0; b < 4; ++b) cb[b] = sc[k][ty + 16 * b];
-#pragma unroll
-            for (int a = 0; a < 4; ++a)
-#pragma unroll
-                for (int b = 0; b < 4; ++b) acc[a][b] += xa[a] * cb[b];
-        }
-        __syncthreads();
-    }
-#pragma unroll
-    for (int b = 0; b < 4; ++b) {
-        const int j = j0 + ty + 16 * b;
-        if (j >= n) continue;
-        double *y = Y + (int64_t)j * ldy;
-        const double bb = beta_mode == 2 ? beta[j] : beta_scalar;
-#pragma unroll
-        for (int a = 0; a < 4; ++a) {
-            const int64_t r = r0 + tx + 16 * a;
-            if (r < nrows) {
-                double v = acc[a][b];
-                if (beta_mode != 0) v += (bb == 0.0 ? 0.0 : bb * y[r]);
-                y[r] = v;
-            }
-        }
-    }
-}
-
-// ---------------------------------------------------------------------
 // misc
 // ---------------------------------------------------------------------
 __device__ __forceinline__ uint64_t splitmix64(uint64_t z)
@@ -579,8 +513,14 @@ void col_dots(int64_t nrows, int ncols, const double *X, int64_t ldx, const doub
     k_sum_partials<<<ncols, 64, 0, s>>>(nb, partial, out);
 }
 
+// Blocks of at least 16 x 16 go to the matrix cores; thinner ones (the
+// one-column products of the column-by-column orthogonalization) to the
+// FMA kernel.
+static bool gram_use_mfma(int m, int n) { return m >= 16 && n >= 16; }
+
 int64_t gram_scratch(int64_t nrows, int m, int n)
 {
+    if (gram_use_mfma(m, n)) return gram_mfma_scratch(nrows, m, n);
     GramPlan p = gram_plan(nrows, m, n);
     return (int64_t)p.nchunk * p.tm * p.tn * kGT * kGT;
 }
@@ -589,6 +529,10 @@ void gram(int64_t nrows, int m, int n, const double *X, int64_t ldx, const doubl
           int64_t ldy, double *G, int ldg, bool lower_only, double *partial, hipStream_t s)
 {
     if (m <= 0 || n <= 0) return;
+    if (gram_use_mfma(m, n)) {
+        gram_mfma(nrows, m, n, X, ldx, Y, ldy, G, ldg, lower_only, partial, s);
+        return;
+    }
     GramPlan p = gram_plan(nrows, m, n);
     const int64_t nblk = (int64_t)p.nchunk * p.tm * p.tn;
     k_gram_partial<<<(unsigned)nblk, kWG, 0, s>>>(nrows, m, n, X, ldx, Y, ldy, p.tm, p.tn,
@@ -598,15 +542,11 @@ void gram(int64_t nrows, int m, int n, const double *X, int64_t ldx, const doubl
                                                         G, ldg);
 }
 
-void lincomb(int64_t nrows, int m, int n, const double *X, int64_t ldx, const double *C,
-             int ldc, int beta_mode, double beta_scalar, const double *beta, double *Y,
+void lincomb(int64_t nrows, int m, int n, const double *X, int64_t ldx, const double *Ct,
+             int ldct, int beta_mode, double beta_scalar, const double *beta, double *Y,
              int64_t ldy, hipStream_t s)
 {
-    if (nrows <= 0 || n <= 0) return;
-    const int ntn = (n + kGT - 1) / kGT;
-    const int64_t ntm = (nrows + kGT - 1) / kGT;
-    k_lincomb<<<(unsigned)(ntm * ntn), kWG, 0, s>>>(nrows, m, n, X, ldx, C, ldc, beta_mode,
-                                                    beta_scalar, beta, Y, ldy, ntn);
+    lincomb_mfma(nrows, m, n, X, ldx, Ct, ldct, beta_mode, beta_scalar, beta, Y, ldy, s);
 }
 
 void random_uniform(int64_t nrows, int ncols, double *Y, int64_t ldy, uint64_t seed,

@@ -1,0 +1,279 @@
+// FP64 matrix-core (MFMA) kernels for the dense contractions of GCG:
+// the Gram blocks of MultiVecInnerProd / MultiVecQtAP (X^T Y) and the
+// tall-skinny products of MultiVecLinearComb (X C).
+//
+// v_mfma_f64_16x16x4_f64 computes D(16x16) += A(16x4) B(4x16) on one
+// wavefront.  Lane l supplies A[l%16][l/16] and B[l/16][l%16] and holds
+// D[l/16 + 4v][l%16] in element v of its 4-double accumulator (a layout
+// any slip in which fails the Gram and LinearComb parity tests of
+// tests/test_gpu_slots.py).
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+#include "kernels.h"
+
+namespace gcge {
+namespace {
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+constexpr int kWG = 256;
+constexpr int kXCD = 8;
+
+__device__ __forceinline__ int64_t xcd_swizzle(int64_t bid, int64_t nblk)
+{
+    const int64_t q = nblk / kXCD, r = nblk % kXCD;
+    const int64_t x = bid % kXCD, k = bid / kXCD;
+    return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + k;
+}
+
+__device__ __forceinline__ d4 mfma(double a, double b, d4 c)
+{
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+// ---------------------------------------------------------------------
+// Gram: G (m x n) = X^T Y.  A workgroup owns a 128 x 128 tile of G over one
+// chunk of rows; its four waves own 64 x 64 quadrants (4 x 4 MFMA tiles).
+// Rows are staged 32 at a time through LDS, stored column-major (row
+// index fastest) so the cooperative global loads and the LDS writes are
+// contiguous, and the MFMA operand reads hit 32 distinct banks per 16
+// lanes.  Partial tiles per chunk are summed by k_gram_mfma_reduce in a
+// fixed order.
+// ---------------------------------------------------------------------
+constexpr int kGT = 128;
+constexpr int kGK = 32;
+constexpr int kGKP = kGK + 1;
+
+__global__ __launch_bounds__(kWG) void k_gram_mfma(
+    int64_t nrows, int m, int n, const double *__restrict__ X, int64_t ldx,
+    const double *__restrict__ Y, int64_t ldy, int tm, int tn, int64_t rows_per_chunk,
+    int lower_only, double *__restrict__ partial)
+{
+    __shared__ double sx[kGT * kGKP];
+    __shared__ double sy[kGT * kGKP];
+    const int64_t lb = xcd_swizzle(blockIdx.x, gridDim.x);
+    const int tiles = tm * tn;
+    const int chunk = (int)(lb / tiles);
+    const int t = (int)(lb % tiles);
+    const int bi = t / tn, bj = t % tn;
+    double *out = partial + ((int64_t)chunk * tiles + t) * (kGT * kGT);
+    if (lower_only && bi < bj) {
+        for (int e = threadIdx.x; e < kGT * kGT; e += kWG) out[e] = 0.0;
+        return;
+    }
+    const int i0 = bi * kGT, j0 = bj * kGT;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int wm = wave >> 1, wn = wave & 1;
+    const int li = lane & 15, lk = lane >> 4;
+    const int64_t rbeg = (int64_t)chunk * rows_per_chunk;
+    int64_t rend = rbeg + rows_per_chunk;
+    if (rend > nrows) rend = nrows;
+    d4 acc[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) acc[a][b] = d4{0.0, 0.0, 0.0, 0.0};
+    for (int64_t r0 = rbeg; r0 < rend; r0 += kGK) {
+#pragma unroll
+        for (int q = 0; q < (kGK * kGT) / kWG; ++q) {
+            const int e = threadIdx.x + q * kWG;
+            const int rr = e % kGK, cc = e / kGK;
+            const int64_t r = r0 + rr;
+            const bool rok = r < rend;
+            sx[cc * kGKP + rr] = (rok && i0 + cc < m) ? X[r + (int64_t)(i0 + cc) * ldx] : 0.0;
+            sy[cc * kGKP + rr] = (rok && j0 + cc < n) ? Y[r + (int64_t)(j0 + cc) * ldy] : 0.0;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int kk = 0; kk < kGK / 4; ++kk) {
+            const int row = kk * 4 + lk;
+            double av[4], bv[4];
+#pragma unroll
+            for (int a = 0; a < 4; ++a) av[a] = sx[(wm * 64 + a * 16 + li) * kGKP + row];
+#pragma unroll
+            for (int b = 0; b < 4; ++b) bv[b] = sy[(wn * 64 + b * 16 + li) * kGKP + row];
+#pragma unroll
+            for (int a = 0; a < 4; ++a)
+#pragma unroll
+                for (int b = 0; b < 4; ++b) acc[a][b] = mfma(av[a], bv[b], acc[a][b]);
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int ii = wm * 64 + a * 16 + lk + 4 * v;
+                const int jj = wn * 64 + b * 16 + li;
+                out[ii * kGT + jj] = acc[a][b][v];
+            }
+}
+
+__global__ __launch_bounds__(kWG) void k_gram_mfma_reduce(int m, int n, int tm, int tn,
+                                                          int nchunk,
+                                                          const double *__restrict__ partial,
+                                                          double *__restrict__ G, int ldg)
+{
+    // threads walk the partial tiles in storage order, so each chunk's read
+    // is contiguous across the wavefront
+    const int64_t off = (int64_t)blockIdx.x * kWG + threadIdx.x;
+    const int64_t stride = (int64_t)tm * tn * kGT * kGT;
+    if (off >= stride) return;
+    const int t = (int)(off / (kGT * kGT));
+    const int within = (int)(off % (kGT * kGT));
+    const int i = (t / tn) * kGT + within / kGT, j = (t % tn) * kGT + within % kGT;
+    if (i >= m || j >= n) return;
+    double s = 0.0;
+    for (int c = 0; c < nchunk; ++c) s += partial[off + (int64_t)c * stride];
+    G[i + (int64_t)j * ldg] = s;
+}
+
+struct GramPlanM {
+    int tm, tn, nchunk;
+    int64_t rows_per_chunk;
+};
+
+GramPlanM plan(int64_t nrows, int m, int n)
+{
+    GramPlanM p;
+    p.tm = (m + kGT - 1) / kGT;
+    p.tn = (n + kGT - 1) / kGT;
+    const int64_t tiles = (int64_t)p.tm * p.tn;
+    int64_t want = (1024 + tiles - 1) / tiles;   // about 4 workgroups per CU
+    int64_t cap = (nrows + 2047) / 2048;         // at least 2048 rows per chunk
+    int64_t nc = want < cap ? want : cap;
+    if (nc < 1) nc = 1;
+    int64_t rpc = (nrows + nc - 1) / nc;
+    rpc = (rpc + kGK - 1) / kGK * kGK;
+    p.rows_per_chunk = rpc;
+    p.nchunk = (int)((nrows + rpc - 1) / rpc);
+    if (p.nchunk < 1) p.nchunk = 1;
+    return p;
+}
+
+// ---------------------------------------------------------------------
+// LinearComb: Y (nrows x n) = X (nrows x m) C (m x n) + Y diag(beta),
+// evaluated as Y^T = C^T X^T so that X feeds the B operand straight from
+// HBM (lane l reads X[i0 + l%16, k + l/16]: four 128-byte row segments per
+// instruction) and the accumulator holds 16 consecutive rows of a column
+// per lane group (contiguous stores).  Ct is C^T packed k-major
+// (Ct[k * ldct + j]).  A wave owns 16 rows and 16*NJ columns.
+// ---------------------------------------------------------------------
+template <int NJ>
+__global__ __launch_bounds__(kWG) void k_lincomb_mfma(
+    int64_t nrows, int m, int n, const double *__restrict__ X, int64_t ldx,
+    const double *__restrict__ Ct, int ldct, int beta_mode, double beta_scalar,
+    const double *__restrict__ beta, double *__restrict__ Y, int64_t ldy, int ntj)
+{
+    const int64_t lb = xcd_swizzle(blockIdx.x, gridDim.x);
+    const int tj = (int)(lb % ntj);
+    const int64_t tb = lb / ntj;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int li = lane & 15, lk = lane >> 4;
+    const int64_t i0 = tb * 64 + wave * 16;
+    const int jbase = tj * 16 * NJ;
+    const int64_t row = i0 + li;
+    const bool rok = row < nrows;
+    d4 acc[NJ];
+#pragma unroll
+    for (int q = 0; q < NJ; ++q) acc[q] = d4{0.0, 0.0, 0.0, 0.0};
+    const double *xr = X + row;
+    int k0 = 0;
+    for (; k0 + 16 <= m; k0 += 16) {
+        double bx[4];
+#pragma unroll
+        for (int s = 0; s < 4; ++s)
+            bx[s] = rok ? xr[(int64_t)(k0 + 4 * s + lk) * ldx] : 0.0;
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            const double *cr = Ct + (int64_t)(k0 + 4 * s + lk) * ldct + jbase + li;
+#pragma unroll
+            for (int q = 0; q < NJ; ++q) {
+                const int j = jbase + q * 16 + li;
+                const double a = (j < n) ? cr[q * 16] : 0.0;
+                acc[q] = mfma(a, bx[s], acc[q]);
+            }
+        }
+    }
+    for (; k0 < m; k0 += 4) {
+        const int k = k0 + lk;
+        const double bx = (rok && k < m) ? xr[(int64_t)k * ldx] : 0.0;
+        const double *cr = Ct + (int64_t)k * ldct + jbase + li;
+#pragma unroll
+        for (int q = 0; q < NJ; ++q) {
+            const int j = jbase + q * 16 + li;
+            const double a = (k < m && j < n) ? cr[q * 16] : 0.0;
+            acc[q] = mfma(a, bx, acc[q]);
+        }
+    }
+    // lane l holds (Y^T)[jbase + q*16 + l/16 + 4v][i0 + l%16]
+    if (!rok) return;
+#pragma unroll
+    for (int q = 0; q < NJ; ++q)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int j = jbase + q * 16 + lk + 4 * v;
+            if (j < n) {
+                double *y = Y + row + (int64_t)j * ldy;
+                double val = acc[q][v];
+                if (beta_mode != 0) {
+                    const double bb = beta_mode == 2 ? beta[j] : beta_scalar;
+                    if (bb != 0.0) val += bb * *y;
+                }
+                *y = val;
+            }
+        }
+}
+
+}  // namespace
+
+int64_t gram_mfma_scratch(int64_t nrows, int m, int n)
+{
+    GramPlanM p = plan(nrows, m, n);
+    return (int64_t)p.nchunk * p.tm * p.tn * kGT * kGT;
+}
+
+void gram_mfma(int64_t nrows, int m, int n, const double *X, int64_t ldx, const double *Y,
+               int64_t ldy, double *G, int ldg, bool lower_only, double *partial,
+               hipStream_t s)
+{
+    if (m <= 0 || n <= 0) return;
+    GramPlanM p = plan(nrows, m, n);
+    const int64_t nblk = (int64_t)p.nchunk * p.tm * p.tn;
+    k_gram_mfma<<<(unsigned)nblk, kWG, 0, s>>>(nrows, m, n, X, ldx, Y, ldy, p.tm, p.tn,
+                                               p.rows_per_chunk, lower_only ? 1 : 0, partial);
+    const int64_t total = (int64_t)p.tm * p.tn * kGT * kGT;
+    k_gram_mfma_reduce<<<(unsigned)((total + kWG - 1) / kWG), kWG, 0, s>>>(
+        m, n, p.tm, p.tn, p.nchunk, partial, G, ldg);
+}
+
+void lincomb_mfma(int64_t nrows, int m, int n, const double *X, int64_t ldx, const double *Ct,
+                  int ldct, int beta_mode, double beta_scalar, const double *beta, double *Y,
+                  int64_t ldy, hipStream_t s)
+{
+    if (nrows <= 0 || n <= 0) return;
+    const int nj16 = (n + 15) / 16;
+    const int64_t nrt = (nrows + 63) / 64;
+#define LAUNCH(NJ)                                                                           \
+    do {                                                                                     \
+        const int ntj = (nj16 + (NJ)-1) / (NJ);                                              \
+        k_lincomb_mfma<NJ><<<(unsigned)(nrt * ntj), kWG, 0, s>>>(                            \
+            nrows, m, n, X, ldx, Ct, ldct, beta_mode, beta_scalar, beta, Y, ldy, ntj);       \
+    } while (0)
+    if (nj16 <= 1)
+        LAUNCH(1);
+    else if (nj16 <= 2)
+        LAUNCH(2);
+    else if (nj16 <= 4)
+        LAUNCH(4);
+    else if (nj16 <= 8)
+        LAUNCH(8);
+    else
+        LAUNCH(16);
+#undef LAUNCH
+}
+
+}  // namespace gcge
