@@ -138,6 +138,7 @@ EXPORTS = {
     "HIPGCG_Free": (None, [OPS_p]),
     "HIPGCG_GetStats": (None, [OPS_p, C.POINTER(C.c_int64), c_dbl_p]),
     "HIPGCG_SetDenseEigensolver": (None, [OPS_p, C.c_int]),
+    "HIPGCG_SetOrthMethod": (None, [OPS_p, C.c_int]),
     "HIPGCG_MultiLinearSolverSetup_BlockPCG32": (None, [C.c_int, C.c_double, C.c_double,
                                                         C.c_void_p, C.c_double, OPS_p]),
 }
@@ -147,6 +148,8 @@ GCGE_HIP_RAND_LIBC = 0
 GCGE_HIP_RAND_DEVICE = 1
 GCGE_EIG_HOST = 0
 GCGE_EIG_DEVICE = 1
+GCGE_ORTH_MGS = 0
+GCGE_ORTH_CHOLQR = 1
 
 _lib = None
 

@@ -48,6 +48,7 @@ struct Solver {
     int compRR_min_num = -1;
     double compRR_min_gap = 0.01, compRR_tol = 1e-16;
     int eig_backend = -1;  // -1: automatic
+    int orth_method = 0;   // GCGE_ORTH_MGS (the reference's) or GCGE_ORTH_CHOLQR
     // run state
     OPS *ops = nullptr;
     void *A = nullptr, *B = nullptr;
@@ -129,31 +130,118 @@ void orth_self(void **x, int start_x, int *end_x, void *B, const OrthParams &p, 
     }
 }
 
+// Removes from x[:, start_x:end_x] its components along x[:, :start_x] in
+// the B inner product: block classical Gram-Schmidt passes until the
+// largest coefficient is below reorth_tol, at most 1 + max_reorth
+// (src/ops_orth.c:231-269).
+void remove_previous(void **x, int start_x, int end_x, void *B, const OrthParams &p,
+                     void **mv_ws, OPS *ops)
+{
+    if (start_x <= 0 || end_x <= start_x) return;
+    int s[2], e[2];
+    double one = 1.0;
+    s[0] = 0; e[0] = start_x;
+    s[1] = start_x; e[1] = end_x;
+    const int64_t len = (int64_t)(e[1] - s[1]) * (e[0] - s[0]);
+    std::vector<double> coef((size_t)len);
+    for (int it = 0; it < 1 + p.max_reorth; ++it) {
+        ops->MultiVecQtAP('S', 'N', x, B, x, 0, s, e, coef.data(), e[0] - s[0], mv_ws, ops);
+        double amax = 0.0;
+        for (int64_t i = 0; i < len; ++i) {
+            coef[i] = -coef[i];
+            amax = std::max(amax, std::fabs(coef[i]));
+        }
+        ops->MultiVecLinearComb(x, x, 0, s, e, coef.data(), e[0] - s[0], &one, 0, ops);
+        if (amax < p.reorth_tol) break;
+    }
+}
+
+void mgs_blocks(void **x, int start_x, int *end_x, void *B, const OrthParams &p, void **mv_ws,
+                OPS *ops);
+
+// Cholesky G = R^T R of an n x n symmetric positive definite matrix
+// (upper R, column-major).  Returns false when a pivot is not above
+// rel_tol times the largest diagonal entry (numerically dependent columns).
+bool cholesky_upper(int n, const std::vector<double> &G, std::vector<double> &R, double rel_tol)
+{
+    R.assign((size_t)n * n, 0.0);
+    double dmax = 0.0;
+    for (int i = 0; i < n; ++i) dmax = std::max(dmax, G[(size_t)n * i + i]);
+    if (!(dmax > 0.0)) return false;
+    for (int j = 0; j < n; ++j) {
+        for (int i = 0; i <= j; ++i) {
+            double s = G[(size_t)n * j + i];
+            for (int k = 0; k < i; ++k) s -= R[(size_t)n * i + k] * R[(size_t)n * j + k];
+            if (i == j) {
+                if (!(s > rel_tol * dmax)) return false;
+                R[(size_t)n * j + j] = std::sqrt(s);
+            } else {
+                R[(size_t)n * j + i] = s / R[(size_t)n * i + i];
+            }
+        }
+    }
+    return true;
+}
+
+// Block orthonormalization of x[:, start_x:*end_x] in the B inner product:
+// the previous columns are removed as in MGS, then Cholesky QR is applied
+// twice to the whole block (CholQR2): G = X^T B X = R^T R, X <- X R^{-1}.
+// The same subspace as the reference's column-by-column MGS with a few
+// block products instead of one B-product and two passes per column.
+// When G is not numerically positive definite (dependent columns) the
+// block falls back to the column-by-column MGS, which drops them.
+void cholqr(void **x, int start_x, int *end_x, void *B, const OrthParams &p, void **mv_ws,
+            OPS *ops)
+{
+    if (*end_x <= start_x) return;
+    remove_previous(x, start_x, *end_x, B, p, mv_ws, ops);
+    const int n = *end_x - start_x;
+    std::vector<double> G((size_t)n * n), R, Rinv((size_t)n * n);
+    for (int pass = 0; pass < 2; ++pass) {
+        int s[2] = {start_x, start_x}, e[2] = {*end_x, *end_x};
+        ops->MultiVecQtAP('S', 'S', x, B, x, 0, s, e, G.data(), n, mv_ws, ops);
+        if (!cholesky_upper(n, G, R, 1e-12)) {
+            mgs_blocks(x, start_x, end_x, B, p, mv_ws, ops);
+            return;
+        }
+        // R^{-1}, upper triangular, by back substitution column by column
+        std::fill(Rinv.begin(), Rinv.end(), 0.0);
+        for (int j = 0; j < n; ++j) {
+            Rinv[(size_t)n * j + j] = 1.0 / R[(size_t)n * j + j];
+            for (int i = j - 1; i >= 0; --i) {
+                double t = 0.0;
+                for (int k = i + 1; k <= j; ++k) t += R[(size_t)n * k + i] * Rinv[(size_t)n * j + k];
+                Rinv[(size_t)n * j + i] = -t / R[(size_t)n * i + i];
+            }
+        }
+        // X <- X R^{-1} through the workspace (the product cannot run in place)
+        int s1[2] = {start_x, 0}, e1[2] = {*end_x, n};
+        ops->MultiVecLinearComb(x, mv_ws, 0, s1, e1, Rinv.data(), n, nullptr, 0, ops);
+        int s2[2] = {0, start_x}, e2[2] = {n, *end_x};
+        ops->MultiVecAxpby(1.0, mv_ws, 0.0, x, s2, e2, ops);
+    }
+}
+
 // Block MGS of x[:, start_x:*end_x] against x[:, :start_x] and itself
 // (src/ops_orth.c:203-394).  mv_ws receives B times the current block.
 void mgs(void **x, int start_x, int *end_x, void *B, const OrthParams &p, void **mv_ws,
          OPS *ops)
 {
     if (*end_x <= start_x) return;
+    remove_previous(x, start_x, *end_x, B, p, mv_ws, ops);
+    mgs_blocks(x, start_x, end_x, B, p, mv_ws, ops);
+}
+
+// The self-orthonormalization part of MGS: blocks of half the range, each
+// orthonormalized column by column and then removed from the rest
+// (src/ops_orth.c:271-368).
+void mgs_blocks(void **x, int start_x, int *end_x, void *B, const OrthParams &p, void **mv_ws,
+                OPS *ops)
+{
     int s[2], e[2];
     double one = 1.0;
     std::vector<double> coef;
-    if (start_x > 0) {
-        s[0] = 0; e[0] = start_x;
-        s[1] = start_x; e[1] = *end_x;
-        const int64_t len = (int64_t)(e[1] - s[1]) * (e[0] - s[0]);
-        coef.resize((size_t)len);
-        for (int it = 0; it < 1 + p.max_reorth; ++it) {
-            ops->MultiVecQtAP('S', 'N', x, B, x, 0, s, e, coef.data(), e[0] - s[0], mv_ws, ops);
-            double amax = 0.0;
-            for (int64_t i = 0; i < len; ++i) {
-                coef[i] = -coef[i];
-                amax = std::max(amax, std::fabs(coef[i]));
-            }
-            ops->MultiVecLinearComb(x, x, 0, s, e, coef.data(), e[0] - s[0], &one, 0, ops);
-            if (amax < p.reorth_tol) break;
-        }
-    }
+    if (*end_x <= start_x) return;
     int init_start = start_x, init_end;
     int bsz = p.block_size;
     if (bsz <= 0) bsz = std::max((*end_x - init_start) / 2, 2);
@@ -332,6 +420,16 @@ struct Timer {
 
 int nV(const Solver &S) { return S.sizeV - S.sizeC; }
 
+// B-orthonormalization of the new W block or the initial X block with the
+// selected method (the reference always uses MGS)
+void orthonormalize(Solver &S, void **V, int start, int *end, const OrthParams &p, void **ws)
+{
+    if (S.orth_method == GCGE_ORTH_CHOLQR)
+        cholqr(V, start, end, S.B, p, ws, S.ops);
+    else
+        mgs(V, start, end, S.B, p, ws, S.ops);
+}
+
 void initialize_x(Solver &S, void **V, void **ritz, int nevGiven)
 {
     Timer t(S, T_INITX);
@@ -341,7 +439,7 @@ void initialize_x(Solver &S, void **V, void **ritz, int nevGiven)
     mgs(V, 0, &nevGiven, S.B, S.orthX, ritz, ops);
     ops->MultiVecSetRandomValue(V, nevGiven, S.sizeX, ops);
     S.endX = S.sizeX;
-    mgs(V, nevGiven, &S.endX, S.B, S.orthX, ritz, ops);
+    orthonormalize(S, V, nevGiven, &S.endX, S.orthX, ritz);
     if (S.endX != S.sizeX) {
         fprintf(stderr, "gcge_hip: initial block lost rank (%d of %d)\n", S.endX, S.sizeX);
         abort();
@@ -510,7 +608,7 @@ void compute_w(Solver &S, void **V, double *ss_eval, void **ritz, const std::vec
             S.linsolves += 1;
         }
     }
-    mgs(V, S.startW, &S.endW, S.B, S.orthW, S.mv_ws[1], ops);
+    orthonormalize(S, V, S.startW, &S.endW, S.orthW, S.mv_ws[1]);
     S.sizeW = S.endW - S.startW;
 }
 
@@ -835,5 +933,7 @@ void HIPGCG_GetStats(OPS *ops, int64_t *stats, double *times)
 }
 
 void HIPGCG_SetDenseEigensolver(OPS *ops, int which) { solver_of(ops)->eig_backend = which; }
+
+void HIPGCG_SetOrthMethod(OPS *ops, int which) { solver_of(ops)->orth_method = which; }
 
 }  // extern "C"

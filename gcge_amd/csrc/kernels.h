@@ -23,15 +23,9 @@ struct SellMat {
     float *val32 = nullptr;
 };
 
-// Y[:, c] = A X[:, c], c < ncols (FP64 / FP32).
+// Y[:, c] = A X[:, c], c < ncols.
 void spmm(const SellMat &A, const double *X, int64_t ldx, double *Y, int64_t ldy,
           int ncols, hipStream_t s);
-void spmm32(const SellMat &A, const float *X, int64_t ldx, float *Y, int64_t ldy,
-            int ncols, hipStream_t s);
-// Y = A X + sigma * B X with A and B sharing the slice structure (FP32).
-void spmm_shift32(const SellMat &A, const SellMat &B, float sigma, const float *X,
-                  int64_t ldx, float *Y, int64_t ldy, int ncols, hipStream_t s);
-
 // Y = alpha X + beta Y over an nrows x ncols block; X null scales only;
 // beta == 0 overwrites Y without reading it.
 void axpby(int64_t nrows, int ncols, double alpha, const double *X, int64_t ldx,

@@ -113,62 +113,6 @@ __global__ __launch_bounds__(kWG) void k_spmm_sell_wide(
     }
 }
 
-// y = A x + sigma B x, A and B with the same slice widths.
-template <typename V, int W, int T>
-__global__ __launch_bounds__(kWG) void k_spmm_shift_sell(
-    int64_t nrows, const int64_t *__restrict__ slice_ptr, const int *__restrict__ colA,
-    const V *__restrict__ valA, const int *__restrict__ colB, const V *__restrict__ valB,
-    V sigma, const V *__restrict__ X, int64_t ldx, V *__restrict__ Y, int64_t ldy,
-    int ncols)
-{
-    const int64_t blk = xcd_swizzle(blockIdx.x, gridDim.x);
-    const int64_t row = blk * kWG + threadIdx.x;
-    if (row >= nrows) return;
-    const int64_t s = row >> 6;
-    const int lane = (int)(row & 63);
-    const int64_t base = slice_ptr[s];
-    const int w = (int)((slice_ptr[s + 1] - base) >> 6);
-    const bool same = (colA == colB);
-    int ca[W], cb[W];
-    V va[W], vb[W];
-#pragma unroll
-    for (int j = 0; j < W; ++j) {
-        if (j < w) {
-            const int64_t e = base + (int64_t)j * 64 + lane;
-            ca[j] = colA[e];
-            va[j] = valA[e];
-            vb[j] = sigma * valB[e];
-            cb[j] = same ? ca[j] : colB[e];
-        } else {
-            ca[j] = cb[j] = (int)row;
-            va[j] = vb[j] = V(0);
-        }
-    }
-    for (int c0 = 0; c0 < ncols; c0 += T) {
-        V acc[T];
-#pragma unroll
-        for (int t = 0; t < T; ++t) acc[t] = V(0);
-#pragma unroll
-        for (int j = 0; j < W; ++j) {
-            if (j < w) {
-#pragma unroll
-                for (int t = 0; t < T; ++t) {
-                    if (c0 + t < ncols) {
-                        const V *xc = X + (int64_t)(c0 + t) * ldx;
-                        if (same)
-                            acc[t] += (va[j] + vb[j]) * xc[ca[j]];
-                        else
-                            acc[t] += va[j] * xc[ca[j]] + vb[j] * xc[cb[j]];
-                    }
-                }
-            }
-        }
-#pragma unroll
-        for (int t = 0; t < T; ++t)
-            if (c0 + t < ncols) Y[row + (int64_t)(c0 + t) * ldy] = acc[t];
-    }
-}
-
 template <typename V, int T>
 void launch_spmm(const SellMat &A, const V *val, const V *X, int64_t ldx, V *Y,
                  int64_t ldy, int ncols, hipStream_t s)
@@ -297,113 +241,119 @@ __global__ __launch_bounds__(64) void k_sum_partials(int nparts,
 }
 
 // ---------------------------------------------------------------------
-// Gram G = X^T Y: 64 x 64 output tiles, each workgroup owns one tile over
-// one chunk of rows and writes a partial tile; a second kernel sums the
-// chunks in a fixed order (deterministic, no atomics).
+// Thin Gram blocks, G (m x n) = X^T Y with n <= 8 (the one-column products
+// of the column-by-column orthogonalization): a GEMV-like stream, one
+// thread per row, MT = 64 / NB columns of X per workgroup and all n columns
+// of Y in registers; X and Y are each read once per column tile.  Partial
+// sums per row chunk are combined by k_gram_thin_reduce in a fixed order.
 // ---------------------------------------------------------------------
-constexpr int kGT = 64;   // output tile
-constexpr int kGK = 32;   // rows per LDS stage
-
-struct GramPlan {
-    int tm, tn, nchunk;
+struct ThinPlan {
+    int nb, mt, mtiles, nchunk;
     int64_t rows_per_chunk;
 };
 
-GramPlan gram_plan(int64_t nrows, int m, int n)
+ThinPlan thin_plan(int64_t nrows, int m, int n)
 {
-    GramPlan p;
-    p.tm = (m + kGT - 1) / kGT;
-    p.tn = (n + kGT - 1) / kGT;
-    const int64_t tiles = (int64_t)p.tm * p.tn;
-    int64_t want = (2048 + tiles - 1) / tiles;
-    int64_t cap = (nrows + 511) / 512;
+    ThinPlan p;
+    p.nb = n <= 1 ? 1 : n <= 2 ? 2 : n <= 4 ? 4 : 8;
+    p.mt = 64 / p.nb;
+    p.mtiles = (m + p.mt - 1) / p.mt;
+    // a few long-lived workgroups per CU: the register accumulators are
+    // reduced once per workgroup, so each should stream many rows
+    int64_t want = (512 + p.mtiles - 1) / p.mtiles;
+    int64_t cap = (nrows + 8191) / 8192;
     int64_t nc = want < cap ? want : cap;
     if (nc < 1) nc = 1;
-    if (nc > 1024) nc = 1024;
-    int64_t rpc = (nrows + nc - 1) / nc;
-    rpc = (rpc + kGK - 1) / kGK * kGK;
-    p.rows_per_chunk = rpc;
-    p.nchunk = (int)((nrows + rpc - 1) / rpc);
+    p.rows_per_chunk = (nrows + nc - 1) / nc;
+    p.nchunk = (int)((nrows + p.rows_per_chunk - 1) / p.rows_per_chunk);
     if (p.nchunk < 1) p.nchunk = 1;
     return p;
 }
 
-__global__ __launch_bounds__(kWG) void k_gram_partial(
-    int64_t nrows, int m, int n, const double *__restrict__ X, int64_t ldx,
-    const double *__restrict__ Y, int64_t ldy, int tm, int tn, int64_t rows_per_chunk,
-    int lower_only, double *__restrict__ partial)
+template <int NB>
+__global__ __launch_bounds__(kWG) void k_gram_thin(int64_t nrows, int m, int n,
+                                                   const double *__restrict__ X, int64_t ldx,
+                                                   const double *__restrict__ Y, int64_t ldy,
+                                                   int64_t rows_per_chunk,
+                                                   double *__restrict__ partial)
 {
-    __shared__ double sx[kGK][kGT + 1];
-    __shared__ double sy[kGK][kGT + 1];
-    const int64_t nblk = (int64_t)gridDim.x;
-    const int64_t lb = xcd_swizzle(blockIdx.x, nblk);
-    const int tiles = tm * tn;
-    const int chunk = (int)(lb / tiles);
-    const int t = (int)(lb % tiles);
-    const int bi = t / tn, bj = t % tn;
-    double *out = partial + ((int64_t)chunk * tiles + t) * (kGT * kGT);
-    const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
-    if (lower_only && bi < bj) {
-        // tile strictly above the diagonal: contributes nothing
-        for (int e = threadIdx.x; e < kGT * kGT; e += kWG) out[e] = 0.0;
-        return;
-    }
-    const int i0 = bi * kGT, j0 = bj * kGT;
+    constexpr int MT = 64 / NB;
+    __shared__ double red[kWG / 64][MT * NB];
+    const int chunk = blockIdx.x, mt = blockIdx.y;
+    const int i0 = mt * MT;
     const int64_t rbeg = (int64_t)chunk * rows_per_chunk;
     int64_t rend = rbeg + rows_per_chunk;
     if (rend > nrows) rend = nrows;
-    double acc[4][4];
+    double acc[MT][NB];
 #pragma unroll
-    for (int a = 0; a < 4; ++a)
+    for (int i = 0; i < MT; ++i)
 #pragma unroll
-        for (int b = 0; b < 4; ++b) acc[a][b] = 0.0;
-    for (int64_t r0 = rbeg; r0 < rend; r0 += kGK) {
-        // stage X[r0:r0+32, i0:i0+64] and Y[r0:r0+32, j0:j0+64]
+        for (int q = 0; q < NB; ++q) acc[i][q] = 0.0;
+    for (int64_t r = rbeg + threadIdx.x; r < rend; r += kWG) {
+        double yv[NB];
 #pragma unroll
-        for (int q = 0; q < (kGK * kGT) / kWG; ++q) {
-            const int e = threadIdx.x + q * kWG;
-            const int rr = e % kGK, cc = e / kGK;
-            const int64_t r = r0 + rr;
-            const bool rok = r < rend;
-            sx[rr][cc] = (rok && i0 + cc < m) ? X[r + (int64_t)(i0 + cc) * ldx] : 0.0;
-            sy[rr][cc] = (rok && j0 + cc < n) ? Y[r + (int64_t)(j0 + cc) * ldy] : 0.0;
+        for (int q = 0; q < NB; ++q) yv[q] = (q < n) ? Y[r + (int64_t)q * ldy] : 0.0;
+#pragma unroll
+        for (int i = 0; i < MT; ++i) {
+            if (i0 + i < m) {
+                const double xv = X[r + (int64_t)(i0 + i) * ldx];
+#pragma unroll
+                for (int q = 0; q < NB; ++q) acc[i][q] += xv * yv[q];
+            }
         }
-        __syncthreads();
-#pragma unroll 8
-        for (int k = 0; k < kGK; ++k) {
-            double xa[4], yb[4];
-#pragma unroll
-            for (int a = 0; a < 4; ++a) xa[a] = sx[k][ty + 16 * a];
-#pragma unroll
-            for (int b = 0; b < 4; ++b) yb[b] = sy[k][tx + 16 * b];
-#pragma unroll
-            for (int a = 0; a < 4; ++a)
-#pragma unroll
-                for (int b = 0; b < 4; ++b) acc[a][b] += xa[a] * yb[b];
-        }
-        __syncthreads();
     }
+    const int wv = threadIdx.x >> 6, ln = threadIdx.x & 63;
 #pragma unroll
-    for (int a = 0; a < 4; ++a)
+    for (int i = 0; i < MT; ++i)
 #pragma unroll
-        for (int b = 0; b < 4; ++b) out[(ty + 16 * a) * kGT + tx + 16 * b] = acc[a][b];
+        for (int q = 0; q < NB; ++q) {
+            const double v = wave_sum(acc[i][q]);
+            if (ln == 0) red[wv][i * NB + q] = v;
+        }
+    __syncthreads();
+    for (int e = threadIdx.x; e < MT * NB; e += kWG) {
+        double s = 0.0;
+#pragma unroll
+        for (int w = 0; w < kWG / 64; ++w) s += red[w][e];
+        partial[((int64_t)chunk * gridDim.y + mt) * (MT * NB) + e] = s;
+    }
 }
 
-__global__ __launch_bounds__(kWG) void k_gram_reduce(int m, int n, int tm, int tn,
-                                                    int nchunk,
-                                                    const double *__restrict__ partial,
-                                                    double *__restrict__ G, int ldg)
+// G[i + j*ldg] (or G[j + i*ldg] when transposed) = sum over chunks
+__global__ __launch_bounds__(kWG) void k_gram_thin_reduce(int m, int n, int mt, int nb,
+                                                         int mtiles, int nchunk,
+                                                         const double *__restrict__ partial,
+                                                         double *__restrict__ G, int ldg,
+                                                         int transposed)
 {
-    const int64_t idx = (int64_t)blockIdx.x * kWG + threadIdx.x;
-    if (idx >= (int64_t)m * n) return;
-    const int i = (int)(idx % m), j = (int)(idx / m);
-    const int bi = i / kGT, bj = j / kGT;
-    const int t = bi * tn + bj;
-    const int64_t off = (int64_t)t * (kGT * kGT) + (i % kGT) * kGT + (j % kGT);
-    const int64_t stride = (int64_t)tm * tn * kGT * kGT;
+    const int64_t off = (int64_t)blockIdx.x * kWG + threadIdx.x;
+    const int64_t stride = (int64_t)mtiles * mt * nb;
+    if (off >= stride) return;
+    const int tile = (int)(off / (mt * nb)), within = (int)(off % (mt * nb));
+    const int i = tile * mt + within / nb, j = within % nb;
+    if (i >= m || j >= n) return;
     double s = 0.0;
     for (int c = 0; c < nchunk; ++c) s += partial[off + (int64_t)c * stride];
-    G[i + (int64_t)j * ldg] = s;
+    if (transposed)
+        G[j + (int64_t)i * ldg] = s;
+    else
+        G[i + (int64_t)j * ldg] = s;
+}
+
+void gram_thin(int64_t nrows, int m, int n, const double *X, int64_t ldx, const double *Y,
+               int64_t ldy, double *G, int ldg, bool transposed, double *partial, hipStream_t s)
+{
+    ThinPlan p = thin_plan(nrows, m, n);
+    dim3 g(p.nchunk, p.mtiles);
+    switch (p.nb) {
+    case 1: k_gram_thin<1><<<g, kWG, 0, s>>>(nrows, m, n, X, ldx, Y, ldy, p.rows_per_chunk, partial); break;
+    case 2: k_gram_thin<2><<<g, kWG, 0, s>>>(nrows, m, n, X, ldx, Y, ldy, p.rows_per_chunk, partial); break;
+    case 4: k_gram_thin<4><<<g, kWG, 0, s>>>(nrows, m, n, X, ldx, Y, ldy, p.rows_per_chunk, partial); break;
+    default: k_gram_thin<8><<<g, kWG, 0, s>>>(nrows, m, n, X, ldx, Y, ldy, p.rows_per_chunk, partial); break;
+    }
+    const int64_t total = (int64_t)p.mtiles * p.mt * p.nb;
+    k_gram_thin_reduce<<<grid1d(total), kWG, 0, s>>>(m, n, p.mt, p.nb, p.mtiles, p.nchunk,
+                                                     partial, G, ldg, transposed ? 1 : 0);
 }
 
 // ---------------------------------------------------------------------
@@ -451,28 +401,7 @@ __global__ __launch_bounds__(kWG) void k_to_f64(int64_t nrows, const float *__re
 void spmm(const SellMat &A, const double *X, int64_t ldx, double *Y, int64_t ldy, int ncols,
           hipStream_t s)
 {
-    launch_spmm<double, 8>(A, A.val, X, ldx, Y, ldy, ncols, s);
-}
-
-void spmm32(const SellMat &A, const float *X, int64_t ldx, float *Y, int64_t ldy, int ncols,
-            hipStream_t s)
-{
-    launch_spmm<float, 16>(A, A.val32, X, ldx, Y, ldy, ncols, s);
-}
-
-void spmm_shift32(const SellMat &A, const SellMat &B, float sigma, const float *X,
-                  int64_t ldx, float *Y, int64_t ldy, int ncols, hipStream_t s)
-{
-    if (ncols <= 0 || A.nrows == 0) return;
-    const unsigned g = grid1d(A.nrows);
-    if (A.max_width <= 8)
-        k_spmm_shift_sell<float, 8, 8><<<g, kWG, 0, s>>>(A.nrows, A.slice_ptr, A.col,
-                                                         A.val32, B.col, B.val32, sigma, X,
-                                                         ldx, Y, ldy, ncols);
-    else
-        k_spmm_shift_sell<float, 32, 4><<<g, kWG, 0, s>>>(A.nrows, A.slice_ptr, A.col,
-                                                          A.val32, B.col, B.val32, sigma, X,
-                                                          ldx, Y, ldy, ncols);
+    launch_spmm<double, 4>(A, A.val, X, ldx, Y, ldy, ncols, s);
 }
 
 void axpby(int64_t nrows, int ncols, double alpha, const double *X, int64_t ldx, double beta,
@@ -513,33 +442,31 @@ void col_dots(int64_t nrows, int ncols, const double *X, int64_t ldx, const doub
     k_sum_partials<<<ncols, 64, 0, s>>>(nb, partial, out);
 }
 
-// Blocks of at least 16 x 16 go to the matrix cores; thinner ones (the
-// one-column products of the column-by-column orthogonalization) to the
-// FMA kernel.
-static bool gram_use_mfma(int m, int n) { return m >= 16 && n >= 16; }
+// Blocks with both sides above 8 go to the matrix cores; thinner ones
+// (the one-column products of the column-by-column orthogonalization) to
+// the streaming kernel, with X and Y swapped when X is the thin side.
+static bool gram_is_thin(int m, int n) { return m <= 8 || n <= 8; }
 
 int64_t gram_scratch(int64_t nrows, int m, int n)
 {
-    if (gram_use_mfma(m, n)) return gram_mfma_scratch(nrows, m, n);
-    GramPlan p = gram_plan(nrows, m, n);
-    return (int64_t)p.nchunk * p.tm * p.tn * kGT * kGT;
+    if (!gram_is_thin(m, n)) return gram_mfma_scratch(nrows, m, n);
+    ThinPlan p = (n <= 8) ? thin_plan(nrows, m, n) : thin_plan(nrows, n, m);
+    return (int64_t)p.nchunk * p.mtiles * p.mt * p.nb;
 }
 
 void gram(int64_t nrows, int m, int n, const double *X, int64_t ldx, const double *Y,
           int64_t ldy, double *G, int ldg, bool lower_only, double *partial, hipStream_t s)
 {
     if (m <= 0 || n <= 0) return;
-    if (gram_use_mfma(m, n)) {
+    if (!gram_is_thin(m, n)) {
         gram_mfma(nrows, m, n, X, ldx, Y, ldy, G, ldg, lower_only, partial, s);
         return;
     }
-    GramPlan p = gram_plan(nrows, m, n);
-    const int64_t nblk = (int64_t)p.nchunk * p.tm * p.tn;
-    k_gram_partial<<<(unsigned)nblk, kWG, 0, s>>>(nrows, m, n, X, ldx, Y, ldy, p.tm, p.tn,
-                                                  p.rows_per_chunk, lower_only ? 1 : 0,
-                                                  partial);
-    k_gram_reduce<<<grid1d((int64_t)m * n), kWG, 0, s>>>(m, n, p.tm, p.tn, p.nchunk, partial,
-                                                        G, ldg);
+    // lower_only needs no special case here: the full block is computed
+    if (n <= 8)
+        gram_thin(nrows, m, n, X, ldx, Y, ldy, G, ldg, false, partial, s);
+    else
+        gram_thin(nrows, n, m, Y, ldy, X, ldx, G, ldg, true, partial, s);
 }
 
 void lincomb(int64_t nrows, int m, int n, const double *X, int64_t ldx, const double *Ct,

@@ -35,15 +35,18 @@ __device__ __forceinline__ d4 mfma(double a, double b, d4 c)
 // ---------------------------------------------------------------------
 // Gram: G (m x n) = X^T Y.  A workgroup owns a 128 x 128 tile of G over one
 // chunk of rows; its four waves own 64 x 64 quadrants (4 x 4 MFMA tiles).
-// Rows are staged 32 at a time through LDS, stored column-major (row
-// index fastest) so the cooperative global loads and the LDS writes are
-// contiguous, and the MFMA operand reads hit 32 distinct banks per 16
-// lanes.  Partial tiles per chunk are summed by k_gram_mfma_reduce in a
-// fixed order.
+// Rows are staged 32 at a time through LDS, stored column-major (row index
+// fastest) so the cooperative global loads and the LDS writes are
+// contiguous and each 16-lane MFMA operand read covers the 32 banks once.
+// The next stage's global loads are issued into registers before the
+// current stage's MFMAs, so HBM latency overlaps the matrix-core work.
+// Partial tiles per chunk are summed by k_gram_mfma_reduce in a fixed
+// order (deterministic, no atomics).
 // ---------------------------------------------------------------------
 constexpr int kGT = 128;
 constexpr int kGK = 32;
 constexpr int kGKP = kGK + 1;
+constexpr int kGL = (kGK * kGT) / kWG;  // elements per thread per operand per stage
 
 __global__ __launch_bounds__(kWG) void k_gram_mfma(
     int64_t nrows, int m, int n, const double *__restrict__ X, int64_t ldx,
@@ -69,22 +72,35 @@ __global__ __launch_bounds__(kWG) void k_gram_mfma(
     const int64_t rbeg = (int64_t)chunk * rows_per_chunk;
     int64_t rend = rbeg + rows_per_chunk;
     if (rend > nrows) rend = nrows;
+    // this thread's slots in a stage: row rr = e % 32, column cc = e / 32
+    const int rr = threadIdx.x % kGK;
+    const int cc0 = threadIdx.x / kGK;  // + 8 * q
+    double px[kGL], py[kGL];
+    auto fetch = [&](int64_t r0) {
+        const int64_t r = r0 + rr;
+        const bool rok = r < rend;
+#pragma unroll
+        for (int q = 0; q < kGL; ++q) {
+            const int cc = cc0 + q * (kWG / kGK);
+            px[q] = (rok && i0 + cc < m) ? X[r + (int64_t)(i0 + cc) * ldx] : 0.0;
+            py[q] = (rok && j0 + cc < n) ? Y[r + (int64_t)(j0 + cc) * ldy] : 0.0;
+        }
+    };
     d4 acc[4][4];
 #pragma unroll
     for (int a = 0; a < 4; ++a)
 #pragma unroll
         for (int b = 0; b < 4; ++b) acc[a][b] = d4{0.0, 0.0, 0.0, 0.0};
+    if (rbeg < rend) fetch(rbeg);
     for (int64_t r0 = rbeg; r0 < rend; r0 += kGK) {
 #pragma unroll
-        for (int q = 0; q < (kGK * kGT) / kWG; ++q) {
-            const int e = threadIdx.x + q * kWG;
-            const int rr = e % kGK, cc = e / kGK;
-            const int64_t r = r0 + rr;
-            const bool rok = r < rend;
-            sx[cc * kGKP + rr] = (rok && i0 + cc < m) ? X[r + (int64_t)(i0 + cc) * ldx] : 0.0;
-            sy[cc * kGKP + rr] = (rok && j0 + cc < n) ? Y[r + (int64_t)(j0 + cc) * ldy] : 0.0;
+        for (int q = 0; q < kGL; ++q) {
+            const int cc = cc0 + q * (kWG / kGK);
+            sx[cc * kGKP + rr] = px[q];
+            sy[cc * kGKP + rr] = py[q];
         }
         __syncthreads();
+        if (r0 + kGK < rend) fetch(r0 + kGK);
 #pragma unroll
         for (int kk = 0; kk < kGK / 4; ++kk) {
             const int row = kk * 4 + lk;
@@ -142,8 +158,8 @@ GramPlanM plan(int64_t nrows, int m, int n)
     p.tm = (m + kGT - 1) / kGT;
     p.tn = (n + kGT - 1) / kGT;
     const int64_t tiles = (int64_t)p.tm * p.tn;
-    int64_t want = (1024 + tiles - 1) / tiles;   // about 4 workgroups per CU
-    int64_t cap = (nrows + 2047) / 2048;         // at least 2048 rows per chunk
+    int64_t want = (512 + tiles - 1) / tiles;   // two workgroups per CU
+    int64_t cap = (nrows + 4095) / 4096;        // at least 4096 rows per chunk
     int64_t nc = want < cap ? want : cap;
     if (nc < 1) nc = 1;
     int64_t rpc = (nrows + nc - 1) / nc;
@@ -156,13 +172,16 @@ GramPlanM plan(int64_t nrows, int m, int n)
 
 // ---------------------------------------------------------------------
 // LinearComb: Y (nrows x n) = X (nrows x m) C (m x n) + Y diag(beta),
-// evaluated as Y^T = C^T X^T so that X feeds the B operand straight from
-// HBM (lane l reads X[i0 + l%16, k + l/16]: four 128-byte row segments per
-// instruction) and the accumulator holds 16 consecutive rows of a column
-// per lane group (contiguous stores).  Ct is C^T packed k-major
-// (Ct[k * ldct + j]).  A wave owns 16 rows and 16*NJ columns.
+// evaluated as Y^T = C^T X^T.  A wave owns 64 rows x 64 columns of Y
+// (4 x 4 MFMA tiles): per step of 4 k it loads four X operands (lane l
+// reads X[i + l%16, k + l/16], 128-byte row segments straight from HBM)
+// and four C^T operands (from L1/L2: C is small) and issues 16 MFMAs, so
+// each operand feeds four matrix-core instructions.  The accumulator
+// holds 16 consecutive rows of one column per lane group, so the stores
+// are contiguous.  Ct is C^T packed k-major: Ct[k * ldct + j].  A
+// workgroup (4 waves) covers 256 rows; consecutive workgroups on one XCD
+// take the column tiles of the same rows.
 // ---------------------------------------------------------------------
-template <int NJ>
 __global__ __launch_bounds__(kWG) void k_lincomb_mfma(
     int64_t nrows, int m, int n, const double *__restrict__ X, int64_t ldx,
     const double *__restrict__ Ct, int ldct, int beta_mode, double beta_scalar,
@@ -173,59 +192,79 @@ __global__ __launch_bounds__(kWG) void k_lincomb_mfma(
     const int64_t tb = lb / ntj;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int li = lane & 15, lk = lane >> 4;
-    const int64_t i0 = tb * 64 + wave * 16;
-    const int jbase = tj * 16 * NJ;
-    const int64_t row = i0 + li;
-    const bool rok = row < nrows;
-    d4 acc[NJ];
+    const int64_t i0 = tb * 256 + wave * 64;
+    const int j0 = tj * 64;
+    bool rok[4];
+    const double *xr[4];
 #pragma unroll
-    for (int q = 0; q < NJ; ++q) acc[q] = d4{0.0, 0.0, 0.0, 0.0};
-    const double *xr = X + row;
-    int k0 = 0;
-    for (; k0 + 16 <= m; k0 += 16) {
-        double bx[4];
+    for (int a = 0; a < 4; ++a) {
+        const int64_t row = i0 + a * 16 + li;
+        rok[a] = row < nrows;
+        xr[a] = X + (rok[a] ? row : 0);
+    }
+    bool jok[4];
 #pragma unroll
-        for (int s = 0; s < 4; ++s)
-            bx[s] = rok ? xr[(int64_t)(k0 + 4 * s + lk) * ldx] : 0.0;
+    for (int q = 0; q < 4; ++q) jok[q] = (j0 + q * 16 + li) < n;
+    d4 acc[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) acc[a][q] = d4{0.0, 0.0, 0.0, 0.0};
+    // k runs in chunks of 16 (four MFMA steps); the next chunk's operands
+    // are loaded while this chunk's 64 MFMAs run, which covers HBM latency
+    // at one wave per SIMD
+    double bx[4][4], ac[4][4], nbx[4][4], nac[4][4];
+    auto load_chunk = [&](int k0) {
 #pragma unroll
         for (int s = 0; s < 4; ++s) {
-            const double *cr = Ct + (int64_t)(k0 + 4 * s + lk) * ldct + jbase + li;
+            const int k = k0 + 4 * s + lk;
+            const bool kok = k < m;
 #pragma unroll
-            for (int q = 0; q < NJ; ++q) {
-                const int j = jbase + q * 16 + li;
-                const double a = (j < n) ? cr[q * 16] : 0.0;
-                acc[q] = mfma(a, bx[s], acc[q]);
+            for (int a = 0; a < 4; ++a)
+                nbx[s][a] = (kok && rok[a]) ? xr[a][(int64_t)k * ldx] : 0.0;
+            const double *cr = Ct + (int64_t)(kok ? k : 0) * ldct + j0 + li;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) nac[s][q] = (kok && jok[q]) ? cr[q * 16] : 0.0;
+        }
+    };
+    load_chunk(0);
+    for (int k0 = 0; k0 < m; k0 += 16) {
+#pragma unroll
+        for (int s = 0; s < 4; ++s)
+#pragma unroll
+            for (int a = 0; a < 4; ++a) {
+                bx[s][a] = nbx[s][a];
+                ac[s][a] = nac[s][a];
             }
-        }
+        if (k0 + 16 < m) load_chunk(k0 + 16);
+#pragma unroll
+        for (int s = 0; s < 4; ++s)
+#pragma unroll
+            for (int a = 0; a < 4; ++a)
+#pragma unroll
+                for (int q = 0; q < 4; ++q) acc[a][q] = mfma(ac[s][q], bx[s][a], acc[a][q]);
     }
-    for (; k0 < m; k0 += 4) {
-        const int k = k0 + lk;
-        const double bx = (rok && k < m) ? xr[(int64_t)k * ldx] : 0.0;
-        const double *cr = Ct + (int64_t)k * ldct + jbase + li;
+    // lane l holds (Y^T)[j0 + q*16 + l/16 + 4v][i0 + a*16 + l%16]
 #pragma unroll
-        for (int q = 0; q < NJ; ++q) {
-            const int j = jbase + q * 16 + li;
-            const double a = (k < m && j < n) ? cr[q * 16] : 0.0;
-            acc[q] = mfma(a, bx, acc[q]);
-        }
-    }
-    // lane l holds (Y^T)[jbase + q*16 + l/16 + 4v][i0 + l%16]
-    if (!rok) return;
+    for (int a = 0; a < 4; ++a) {
+        if (!rok[a]) continue;
+        const int64_t row = i0 + a * 16 + li;
 #pragma unroll
-    for (int q = 0; q < NJ; ++q)
+        for (int q = 0; q < 4; ++q)
 #pragma unroll
-        for (int v = 0; v < 4; ++v) {
-            const int j = jbase + q * 16 + lk + 4 * v;
-            if (j < n) {
-                double *y = Y + row + (int64_t)j * ldy;
-                double val = acc[q][v];
-                if (beta_mode != 0) {
-                    const double bb = beta_mode == 2 ? beta[j] : beta_scalar;
-                    if (bb != 0.0) val += bb * *y;
+            for (int v = 0; v < 4; ++v) {
+                const int j = j0 + q * 16 + lk + 4 * v;
+                if (j < n) {
+                    double *y = Y + row + (int64_t)j * ldy;
+                    double val = acc[a][q][v];
+                    if (beta_mode != 0) {
+                        const double bb = beta_mode == 2 ? beta[j] : beta_scalar;
+                        if (bb != 0.0) val += bb * *y;
+                    }
+                    *y = val;
                 }
-                *y = val;
             }
-        }
+    }
 }
 
 }  // namespace
@@ -255,25 +294,11 @@ void lincomb_mfma(int64_t nrows, int m, int n, const double *X, int64_t ldx, con
                   int64_t ldy, hipStream_t s)
 {
     if (nrows <= 0 || n <= 0) return;
-    const int nj16 = (n + 15) / 16;
-    const int64_t nrt = (nrows + 63) / 64;
-#define LAUNCH(NJ)                                                                           \
-    do {                                                                                     \
-        const int ntj = (nj16 + (NJ)-1) / (NJ);                                              \
-        k_lincomb_mfma<NJ><<<(unsigned)(nrt * ntj), kWG, 0, s>>>(                            \
-            nrows, m, n, X, ldx, Ct, ldct, beta_mode, beta_scalar, beta, Y, ldy, ntj);       \
-    } while (0)
-    if (nj16 <= 1)
-        LAUNCH(1);
-    else if (nj16 <= 2)
-        LAUNCH(2);
-    else if (nj16 <= 4)
-        LAUNCH(4);
-    else if (nj16 <= 8)
-        LAUNCH(8);
-    else
-        LAUNCH(16);
-#undef LAUNCH
+    const int ntj = (n + 63) / 64;
+    const int64_t nrt = (nrows + 255) / 256;
+    k_lincomb_mfma<<<(unsigned)(nrt * ntj), kWG, 0, s>>>(nrows, m, n, X, ldx, Ct, ldct,
+                                                        beta_mode, beta_scalar, beta, Y, ldy,
+                                                        ntj);
 }
 
 }  // namespace gcge

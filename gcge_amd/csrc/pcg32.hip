@@ -5,7 +5,7 @@
 // It solves (A + sigma B) x_c = b_c for the columns of a block with the
 // stopping rule of BlockPCG (src/ops_lin_sol.c:383-394): a column stops
 // once its residual is below rate * initial residual or below tol, or
-// after max_iter iterations.
+// after max_iter iterations; a stopped column's iterate is frozen.
 //
 // One call is one step of mixed-precision iterative refinement: the
 // residual r = b - op(x) is formed in FP64 (k_resid64), the correction d
@@ -13,15 +13,19 @@
 // tolerates the inexact inner solve; its FP64 residual check and
 // Rayleigh-Ritz step refine further.
 //
-// Per iteration, for the active columns only (their indices are compacted
-// on the device):
-//   k_update_p     p = r + beta p                       (beta = rho/rho_old)
+// The FP32 work vectors (d, r, p, w) are private, so they use a layout made
+// for the stencil gather: columns in tiles of 8, interleaved by row,
+// element (row, col) at ((col / 8) * n + row) * 8 + col % 8.  A neighbour
+// row's 8 values are one contiguous 32-byte read per lane (2 KB per
+// wavefront).  Per iteration, over the tiles that still hold an active
+// column (their list is compacted on the device):
+//   k_update_p     p = r + beta p
 //   k_spmm_pw      w = (A + sigma B) p, partial p.w per workgroup
-//   k_finish_pw    alpha = rho / p.w
-//   k_update_xr    x += alpha p, r -= alpha w, partial r.r per workgroup
-//   k_finish_rr    rho_new, convergence flags, compacted active list
-// and one 4-byte read of the active count by the host.  All reductions are
-// written per workgroup and summed in a fixed order (no atomics).
+//   k_finish_pw    alpha = rho / p.w (0 for stopped columns)
+//   k_update_xr    d += alpha p, r -= alpha w, partial r.r per workgroup
+//   k_finish_rr    rho, stop flags, compacted tile list
+// and one 4-byte read of the active tile count by the host.  Reductions
+// are written per workgroup and summed in a fixed order (no atomics).
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -38,7 +42,11 @@ namespace {
 
 constexpr int kWG = 256;
 constexpr int kXCD = 8;
-constexpr int kT = 8;  // columns per register tile
+constexpr int kC = 8;  // columns per interleaved tile
+
+struct alignas(32) f8 {
+    float v[kC];
+};
 
 __device__ __forceinline__ int64_t xcd_swizzle(int64_t bid, int64_t nblk)
 {
@@ -54,45 +62,55 @@ __device__ __forceinline__ float wave_sum(float v)
     return v;
 }
 
-// Sum over the workgroup of kT per-thread values; results in red[0..kT).
-__device__ __forceinline__ void block_sum_tile(float (&v)[kT], float *red /* [4][kT] */)
+// Sum over the workgroup of kC per-thread values; result in red[4*kC + t].
+__device__ __forceinline__ void block_sum8(const float (&v)[kC], float *red)
 {
     const int wv = threadIdx.x >> 6, ln = threadIdx.x & 63;
 #pragma unroll
-    for (int t = 0; t < kT; ++t) {
+    for (int t = 0; t < kC; ++t) {
         const float s = wave_sum(v[t]);
-        if (ln == 0) red[wv * kT + t] = s;
+        if (ln == 0) red[wv * kC + t] = s;
     }
     __syncthreads();
-    if (threadIdx.x < kT) {
+    if (threadIdx.x < kC) {
         float s = 0.f;
 #pragma unroll
-        for (int k = 0; k < kWG / 64; ++k) s += red[k * kT + threadIdx.x];
-        red[kWG / 64 * kT + threadIdx.x] = s;
+        for (int k = 0; k < kWG / 64; ++k) s += red[k * kC + threadIdx.x];
+        red[kWG / 64 * kC + threadIdx.x] = s;
     }
     __syncthreads();
 }
 
-// Residual of the FP64 system, rounded to FP32: for the columns c < nc,
-// R32[:, c] = fl32(b[:, c] - (A + sigma B) x[:, c]) with the products in
-// FP64 from the FP64 matrix values, and partial[c * nblk + block] = the
-// block's sum of R32^2.  This is the refinement step: the FP32 solve then
-// finds the correction, whose size is that of the residual, instead of
-// the whole solution, whose FP32 rounding would swamp the correction.
-template <int W>
+__device__ __forceinline__ f8 load8(const float *p)
+{
+    return *reinterpret_cast<const f8 *>(p);
+}
+
+__device__ __forceinline__ void store8(float *p, const f8 &x)
+{
+    *reinterpret_cast<f8 *>(p) = x;
+}
+
+// Residual of the FP64 system, rounded to FP32, for every column c < nc:
+// R[c] = fl32(b[:, c] - (A + sigma B) x[:, c]) with the products in FP64
+// from the FP64 matrix values; partial[c * nblk + block] = sum of R^2.
+// Columns nc .. 8*ntile-1 of the last tile are set to zero.  This is the
+// refinement step: the FP32 solve then finds the correction, whose size
+// is that of the residual, not the whole solution, whose FP32 rounding
+// would swamp the correction.
+template <int W, bool SHIFT>
 __global__ __launch_bounds__(kWG) void k_resid64(
     int64_t n, const int64_t *__restrict__ slice_ptr, const int *__restrict__ colA,
     const double *__restrict__ valA, const int *__restrict__ colB,
     const double *__restrict__ valB, double sigma, int nc, const double *__restrict__ Bv,
-    int64_t ldb, const double *__restrict__ Xv, int64_t ldx, float *__restrict__ R32,
-    int64_t ldr, float *__restrict__ partial)
+    int64_t ldb, const double *__restrict__ Xv, int64_t ldx, float *__restrict__ R,
+    float *__restrict__ partial)
 {
-    __shared__ float red[(kWG / 64 + 1) * kT];
+    __shared__ float red[(kWG / 64 + 1) * kC];
     const int64_t nblk = gridDim.x;
     const int64_t blk = xcd_swizzle(blockIdx.x, nblk);
     const int64_t row = blk * kWG + threadIdx.x;
     const bool live = row < n;
-    const bool shifted = (sigma != 0.0) && (valB != nullptr);
     int ca[W], cb[W];
     double va[W], vb[W];
     int w = 0;
@@ -107,74 +125,87 @@ __global__ __launch_bounds__(kWG) void k_resid64(
                 const int64_t e = base + (int64_t)j * 64 + lane;
                 ca[j] = colA[e];
                 va[j] = valA[e];
-                cb[j] = shifted ? colB[e] : ca[j];
-                vb[j] = shifted ? sigma * valB[e] : 0.0;
+                cb[j] = SHIFT ? colB[e] : ca[j];
+                vb[j] = SHIFT ? sigma * valB[e] : 0.0;
             } else {
                 ca[j] = cb[j] = (int)row;
                 va[j] = vb[j] = 0.0;
             }
         }
     }
-    for (int c0 = 0; c0 < nc; c0 += kT) {
-        float sq[kT];
+    const int ntile = (nc + kC - 1) / kC;
+    for (int tile = 0; tile < ntile; ++tile) {
+        float sq[kC];
+        f8 r8;
 #pragma unroll
-        for (int t = 0; t < kT; ++t) sq[t] = 0.f;
+        for (int t = 0; t < kC; ++t) {
+            sq[t] = 0.f;
+            r8.v[t] = 0.f;
+        }
         if (live) {
 #pragma unroll
-            for (int t = 0; t < kT; ++t) {
-                if (c0 + t < nc) {
-                    const double *xc = Xv + (int64_t)(c0 + t) * ldx;
+            for (int t = 0; t < kC; ++t) {
+                const int c = tile * kC + t;
+                if (c < nc) {
+                    const double *xc = Xv + (int64_t)c * ldx;
                     double acc = 0.0;
 #pragma unroll
                     for (int j = 0; j < W; ++j)
-                        if (j < w) acc += va[j] * xc[ca[j]] + vb[j] * xc[cb[j]];
-                    const float r = (float)(Bv[row + (int64_t)(c0 + t) * ldb] - acc);
-                    R32[row + (int64_t)(c0 + t) * ldr] = r;
+                        if (j < w) {
+                            acc += va[j] * xc[ca[j]];
+                            if (SHIFT) acc += vb[j] * xc[cb[j]];
+                        }
+                    const float r = (float)(Bv[row + (int64_t)c * ldb] - acc);
+                    r8.v[t] = r;
                     sq[t] = r * r;
                 }
             }
+            store8(R + ((int64_t)tile * n + row) * kC, r8);
         }
-        block_sum_tile(sq, red);
-        if (threadIdx.x < kT && c0 + (int)threadIdx.x < nc)
-            partial[(int64_t)(c0 + threadIdx.x) * nblk + blockIdx.x] =
-                red[kWG / 64 * kT + threadIdx.x];
+        block_sum8(sq, red);
+        if (threadIdx.x < kC)
+            partial[(int64_t)(tile * kC + threadIdx.x) * nblk + blockIdx.x] =
+                red[kWG / 64 * kC + threadIdx.x];
         __syncthreads();
     }
 }
 
-// x[:, c] += (double) d[:, c]  and  d[:, c] = 0 (ready for the next solve)
-__global__ __launch_bounds__(kWG) void k_add_up(int64_t n, float *__restrict__ D, int64_t ldd,
+// x[:, c] += d[c] in FP64, for c < nc
+__global__ __launch_bounds__(kWG) void k_add_up(int64_t n, int nc, const float *__restrict__ D,
                                                 double *__restrict__ Xv, int64_t ldx)
 {
     const int c = blockIdx.y;
+    if (c >= nc) return;
+    const int64_t r = (int64_t)blockIdx.x * kWG + threadIdx.x;
+    if (r < n) Xv[r + (int64_t)c * ldx] += (double)D[((int64_t)(c / kC) * n + r) * kC + c % kC];
+}
+
+// zero the tiles of d
+__global__ __launch_bounds__(kWG) void k_zero_tiles(int64_t n, float *__restrict__ D)
+{
+    const int tile = blockIdx.y;
     const int64_t r = (int64_t)blockIdx.x * kWG + threadIdx.x;
     if (r < n) {
-        Xv[r + (int64_t)c * ldx] += (double)D[r + (int64_t)c * ldd];
-        D[r + (int64_t)c * ldd] = 0.f;
+        f8 z;
+#pragma unroll
+        for (int t = 0; t < kC; ++t) z.v[t] = 0.f;
+        store8(D + ((int64_t)tile * n + r) * kC, z);
     }
 }
 
-__global__ __launch_bounds__(kWG) void k_zero32(int64_t n, float *__restrict__ D, int64_t ldd)
-{
-    const int c = blockIdx.y;
-    const int64_t r = (int64_t)blockIdx.x * kWG + threadIdx.x;
-    if (r < n) D[r + (int64_t)c * ldd] = 0.f;
-}
-
-// Shifted SpMM on the listed columns with the p.w partial sums fused.
-// w[:, c] = (A + sigma B) p[:, c] for c = cols[k], k < nact;
-// partial[k * nblk + block] = sum over the block's rows of p * w.
-// When with_dot == 0 the partial sums are skipped (initial residual).
+// Shifted SpMM on the listed tiles with the p.w partial sums fused:
+// w = (A + sigma B) p for the 8 columns of tile tiles[k], k < ntact;
+// partial[(tile*8 + t) * nblk + block] = the block's sum of p * w.
 template <int W>
 __global__ __launch_bounds__(kWG) void k_spmm_pw(
     int64_t n, const int64_t *__restrict__ slice_ptr, const int *__restrict__ colA,
-    const float *__restrict__ valA, const int *__restrict__ colB, const float *__restrict__ valB,
-    float sigma, const int *__restrict__ cols, const int *__restrict__ nact_p,
-    const float *__restrict__ P, float *__restrict__ Wv, int64_t ld, float *__restrict__ partial,
-    int with_dot)
+    const float *__restrict__ valA, const int *__restrict__ colB,
+    const float *__restrict__ valB, float sigma, const int *__restrict__ tiles,
+    const int *__restrict__ ntact_p, const float *__restrict__ P, float *__restrict__ Wv,
+    float *__restrict__ partial)
 {
-    __shared__ float red[(kWG / 64 + 1) * kT];
-    const int nact = *nact_p;
+    __shared__ float red[(kWG / 64 + 1) * kC];
+    const int ntact = *ntact_p;
     const int64_t nblk = gridDim.x;
     const int64_t blk = xcd_swizzle(blockIdx.x, nblk);
     const int64_t row = blk * kWG + threadIdx.x;
@@ -196,181 +227,197 @@ __global__ __launch_bounds__(kWG) void k_spmm_pw(
                 va[j] = valA[e];
                 vb[j] = sigma * valB[e];
                 cb[j] = same ? ca[j] : colB[e];
+                if (same) {
+                    va[j] += vb[j];
+                    vb[j] = 0.f;
+                }
             } else {
                 ca[j] = cb[j] = (int)row;
                 va[j] = vb[j] = 0.f;
             }
         }
     }
-    for (int k0 = 0; k0 < nact; k0 += kT) {
-        float acc[kT], dot[kT];
+    for (int k = 0; k < ntact; ++k) {
+        const int tile = tiles[k];
+        const float *pt = P + (int64_t)tile * n * kC;
+        float dot[kC];
+        f8 acc;
 #pragma unroll
-        for (int t = 0; t < kT; ++t) {
-            acc[t] = 0.f;
+        for (int t = 0; t < kC; ++t) {
+            acc.v[t] = 0.f;
             dot[t] = 0.f;
         }
         if (live) {
-            int64_t off[kT];
-#pragma unroll
-            for (int t = 0; t < kT; ++t)
-                off[t] = (k0 + t < nact) ? (int64_t)cols[k0 + t] * ld : -1;
 #pragma unroll
             for (int j = 0; j < W; ++j) {
                 if (j < w) {
+                    const f8 pa = load8(pt + (int64_t)ca[j] * kC);
 #pragma unroll
-                    for (int t = 0; t < kT; ++t) {
-                        if (off[t] >= 0) {
-                            const float *pc = P + off[t];
-                            if (same)
-                                acc[t] += (va[j] + vb[j]) * pc[ca[j]];
-                            else
-                                acc[t] += va[j] * pc[ca[j]] + vb[j] * pc[cb[j]];
-                        }
+                    for (int t = 0; t < kC; ++t) acc.v[t] += va[j] * pa.v[t];
+                    if (!same) {
+                        const f8 pb = load8(pt + (int64_t)cb[j] * kC);
+#pragma unroll
+                        for (int t = 0; t < kC; ++t) acc.v[t] += vb[j] * pb.v[t];
                     }
                 }
             }
+            store8(Wv + ((int64_t)tile * n + row) * kC, acc);
+            const f8 pr = load8(pt + row * kC);
 #pragma unroll
-            for (int t = 0; t < kT; ++t) {
-                if (off[t] >= 0) {
-                    Wv[off[t] + row] = acc[t];
-                    if (with_dot) dot[t] = acc[t] * P[off[t] + row];
-                }
-            }
+            for (int t = 0; t < kC; ++t) dot[t] = acc.v[t] * pr.v[t];
         }
-        if (with_dot) {
-            block_sum_tile(dot, red);
-            if (threadIdx.x < kT && k0 + (int)threadIdx.x < nact)
-                partial[(int64_t)(k0 + threadIdx.x) * nblk + blockIdx.x] =
-                    red[kWG / 64 * kT + threadIdx.x];
-            __syncthreads();
-        }
+        block_sum8(dot, red);
+        if (threadIdx.x < kC)
+            partial[(int64_t)(tile * kC + threadIdx.x) * nblk + blockIdx.x] =
+                red[kWG / 64 * kC + threadIdx.x];
+        __syncthreads();
     }
 }
 
-// p = r + beta p on the active columns; beta = 0 on the first iteration.
-__global__ __launch_bounds__(kWG) void k_update_p(int64_t n, const int *__restrict__ cols,
-                                                  const int *__restrict__ nact_p,
+// p = r + beta p on the listed tiles; p = r on the first iteration (p is
+// overwritten without being read: the workspace may hold anything)
+__global__ __launch_bounds__(kWG) void k_update_p(int64_t n, const int *__restrict__ tiles,
+                                                  const int *__restrict__ ntact_p,
                                                   const float *__restrict__ rho,
                                                   const float *__restrict__ rho_old, int first,
                                                   const float *__restrict__ R,
-                                                  float *__restrict__ P, int64_t ld)
+                                                  float *__restrict__ P)
 {
     const int k = blockIdx.y;
-    if (k >= *nact_p) return;
-    const int c = cols[k];
-    const float beta = first ? 0.f : rho[c] / rho_old[c];
-    const float *r = R + (int64_t)c * ld;
-    float *p = P + (int64_t)c * ld;
-    // the first iteration overwrites p without reading it: the workspace
-    // may hold anything, including NaN
-    if (first) {
-        for (int64_t i = (int64_t)blockIdx.x * kWG + threadIdx.x; i < n;
-             i += (int64_t)gridDim.x * kWG)
-            p[i] = r[i];
-    } else {
-        for (int64_t i = (int64_t)blockIdx.x * kWG + threadIdx.x; i < n;
-             i += (int64_t)gridDim.x * kWG)
-            p[i] = r[i] + beta * p[i];
+    if (k >= *ntact_p) return;
+    const int tile = tiles[k];
+    float beta[kC];
+#pragma unroll
+    for (int t = 0; t < kC; ++t) {
+        const int c = tile * kC + t;
+        beta[t] = (first || rho_old[c] == 0.f) ? 0.f : rho[c] / rho_old[c];
+    }
+    const int64_t o = (int64_t)tile * n * kC;
+    for (int64_t i = (int64_t)blockIdx.x * kWG + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * kWG) {
+        const f8 r = load8(R + o + i * kC);
+        f8 p;
+        if (first) {
+            p = r;
+        } else {
+            p = load8(P + o + i * kC);
+#pragma unroll
+            for (int t = 0; t < kC; ++t) p.v[t] = r.v[t] + beta[t] * p.v[t];
+        }
+        store8(P + o + i * kC, p);
     }
 }
 
-// x += alpha p, r -= alpha w on the active columns, with r.r partial sums.
-__global__ __launch_bounds__(kWG) void k_update_xr(int64_t n, const int *__restrict__ cols,
-                                                   const int *__restrict__ nact_p,
+// d += alpha p, r -= alpha w on the listed tiles, with r.r partial sums
+__global__ __launch_bounds__(kWG) void k_update_xr(int64_t n, const int *__restrict__ tiles,
+                                                   const int *__restrict__ ntact_p,
                                                    const float *__restrict__ alpha,
                                                    const float *__restrict__ P,
                                                    const float *__restrict__ Wv,
-                                                   float *__restrict__ X, float *__restrict__ R,
-                                                   int64_t ld, float *__restrict__ partial)
+                                                   float *__restrict__ D, float *__restrict__ R,
+                                                   float *__restrict__ partial)
 {
-    __shared__ float red[kWG / 64];
+    __shared__ float red[(kWG / 64 + 1) * kC];
     const int k = blockIdx.y;
-    if (k >= *nact_p) return;
-    const int c = cols[k];
-    const float a = alpha[c];
-    const int64_t o = (int64_t)c * ld;
-    float s = 0.f;
+    if (k >= *ntact_p) return;
+    const int tile = tiles[k];
+    float a[kC], s[kC];
+#pragma unroll
+    for (int t = 0; t < kC; ++t) {
+        a[t] = alpha[tile * kC + t];
+        s[t] = 0.f;
+    }
+    const int64_t o = (int64_t)tile * n * kC;
     for (int64_t i = (int64_t)blockIdx.x * kWG + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * kWG) {
-        X[o + i] += a * P[o + i];
-        const float rv = R[o + i] - a * Wv[o + i];
-        R[o + i] = rv;
-        s += rv * rv;
-    }
-    s = wave_sum(s);
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        float t = 0.f;
+        const f8 p = load8(P + o + i * kC), w = load8(Wv + o + i * kC);
+        f8 d = load8(D + o + i * kC), r = load8(R + o + i * kC);
 #pragma unroll
-        for (int q = 0; q < kWG / 64; ++q) t += red[q];
-        partial[(int64_t)k * gridDim.x + blockIdx.x] = t;
+        for (int t = 0; t < kC; ++t) {
+            d.v[t] += a[t] * p.v[t];
+            r.v[t] -= a[t] * w.v[t];
+            s[t] += r.v[t] * r.v[t];
+        }
+        store8(D + o + i * kC, d);
+        store8(R + o + i * kC, r);
     }
+    block_sum8(s, red);
+    if (threadIdx.x < kC)
+        partial[(int64_t)(tile * kC + threadIdx.x) * gridDim.x + blockIdx.x] =
+            red[kWG / 64 * kC + threadIdx.x];
 }
 
-// alpha[c] = rho[c] / sum(partial) for the active columns (one wave each)
-__global__ __launch_bounds__(64) void k_finish_pw(const int *__restrict__ cols,
-                                                  const int *__restrict__ nact_p, int nparts,
+// alpha[c] = rho[c] / (p.w)[c] for the active columns of the listed tiles,
+// 0 for stopped ones (their iterate stays frozen).  One wave per column.
+__global__ __launch_bounds__(64) void k_finish_pw(const int *__restrict__ tiles,
+                                                  const int *__restrict__ ntact_p, int nparts,
                                                   const float *__restrict__ partial,
                                                   const float *__restrict__ rho,
+                                                  const int *__restrict__ active,
                                                   float *__restrict__ alpha)
 {
-    const int k = blockIdx.x;
-    if (k >= *nact_p) return;
+    const int k = blockIdx.x / kC, t = blockIdx.x % kC;
+    if (k >= *ntact_p) return;
+    const int c = tiles[k] * kC + t;
     double s = 0.0;
-    for (int i = threadIdx.x; i < nparts; i += 64) s += partial[(int64_t)k * nparts + i];
+    for (int i = threadIdx.x; i < nparts; i += 64) s += partial[(int64_t)c * nparts + i];
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
-    if (threadIdx.x == 0) {
-        const int c = cols[k];
-        alpha[c] = (float)((double)rho[c] / s);
-    }
+    if (threadIdx.x == 0) alpha[c] = active[c] ? (float)((double)rho[c] / s) : 0.f;
 }
 
-// New residual norms, convergence and the compacted active list; a single
-// workgroup handles every column (<= 1024).  mode 0: initial residual
-// (all columns listed in order), mode 1: iteration.
-__global__ __launch_bounds__(1024) void k_finish_rr(int ncols, int mode, int nparts,
+// New residual norms, stop flags and the compacted tile list; one
+// workgroup, one thread per column (<= 1024).  mode 0: initial residual
+// of every column c < nc; mode 1: after an iteration, over the active
+// columns.
+__global__ __launch_bounds__(1024) void k_finish_rr(int nc, int mode, int nparts,
                                                     const float *__restrict__ partial,
-                                                    int *__restrict__ cols,
-                                                    int *__restrict__ nact_p,
+                                                    int *__restrict__ tiles,
+                                                    int *__restrict__ ntact_p,
+                                                    int *__restrict__ active,
                                                     float *__restrict__ rho,
                                                     float *__restrict__ rho_old,
                                                     float *__restrict__ init_res, float rate,
                                                     float tol)
 {
-    __shared__ int keep[1024];
-    __shared__ int pos[1024];
-    const int k = threadIdx.x;
-    const int nact = (mode == 0) ? ncols : *nact_p;
-    int c = -1, kp = 0;
-    if (k < nact) {
-        c = (mode == 0) ? k : cols[k];
-        double s = 0.0;
-        for (int i = 0; i < nparts; ++i) s += partial[(int64_t)k * nparts + i];
-        const float res = (float)std::sqrt(s);
-        if (mode == 0) {
-            init_res[c] = res;
-            rho[c] = (float)s;
-            kp = res > tol ? 1 : 0;
-        } else {
-            rho_old[c] = rho[c];
-            rho[c] = (float)s;
-            kp = (res > rate * init_res[c] && res > tol) ? 1 : 0;
-        }
-    }
-    keep[k] = kp;
+    __shared__ int tile_on[1024 / kC];
+    const int c = threadIdx.x;
+    const int ntile = (nc + kC - 1) / kC;
+    if (c < ntile) tile_on[c] = 0;
     __syncthreads();
-    if (k == 0) {
+    if (c < nc) {
+        const bool was = (mode == 0) ? true : active[c] != 0;
+        if (was) {
+            double s = 0.0;
+            for (int i = 0; i < nparts; ++i) s += partial[(int64_t)c * nparts + i];
+            const float res = (float)std::sqrt(s);
+            int on;
+            if (mode == 0) {
+                init_res[c] = res;
+                rho_old[c] = 0.f;
+                rho[c] = (float)s;
+                on = res > tol ? 1 : 0;
+            } else {
+                rho_old[c] = rho[c];
+                rho[c] = (float)s;
+                on = (res > rate * init_res[c] && res > tol) ? 1 : 0;
+            }
+            active[c] = on;
+            if (on) tile_on[c / kC] = 1;  // benign race: all writers store 1
+        }
+    } else if (c < ntile * kC && mode == 0) {
+        // padding columns of the last tile: inactive, with rho = 0 so that
+        // beta = 0 and their p, w, d and r stay exactly zero
+        active[c] = 0;
+        rho[c] = rho_old[c] = init_res[c] = 0.f;
+    }
+    __syncthreads();
+    if (c == 0) {
         int q = 0;
-        for (int i = 0; i < nact; ++i) {
-            pos[i] = q;
-            q += keep[i];
-        }
-        *nact_p = q;
+        for (int t = 0; t < ntile; ++t)
+            if (tile_on[t]) tiles[q++] = t;
+        *ntact_p = q;
     }
-    __syncthreads();
-    if (k < nact && kp) cols[pos[k]] = c;
 }
 
 struct PCG32 {
@@ -378,30 +425,34 @@ struct PCG32 {
     double rate = 1e-2, tol = 1e-14;
     void *B = nullptr;
     double sigma = 0.0;
-    // device workspace, FP32, ncap columns of n rows each
+    // device workspace: ncap columns (a multiple of 8) of n rows each
     int64_t n = 0;
     int ncap = 0;
     float *x = nullptr, *r = nullptr, *p = nullptr, *w = nullptr;
     float *rho = nullptr, *rho_old = nullptr, *init_res = nullptr, *alpha = nullptr;
     float *partial = nullptr;
-    int *cols = nullptr, *nact = nullptr;
-    int *h_nact = nullptr;
+    int *tiles = nullptr, *ntact = nullptr, *active = nullptr;
+    int *h_ntact = nullptr;
     int64_t iters = 0;
     void ensure(int64_t nrows, int nc)
     {
-        if (nrows == n && nc <= ncap) return;
+        const int nc8 = (nc + kC - 1) / kC * kC;
+        if (nrows == n && nc8 <= ncap) return;
         release();
         n = nrows;
-        ncap = nc;
-        const size_t blkbytes = sizeof(float) * (size_t)n * nc;
-        for (float **q : {&x, &r, &p, &w}) HIPCHECK(hipMalloc(reinterpret_cast<void **>(q), blkbytes));
+        ncap = nc8;
+        const size_t blkbytes = sizeof(float) * (size_t)n * ncap;
+        for (float **q : {&x, &r, &p, &w})
+            HIPCHECK(hipMalloc(reinterpret_cast<void **>(q), blkbytes));
         for (float **q : {&rho, &rho_old, &init_res, &alpha})
-            HIPCHECK(hipMalloc(reinterpret_cast<void **>(q), sizeof(float) * nc));
+            HIPCHECK(hipMalloc(reinterpret_cast<void **>(q), sizeof(float) * ncap));
         HIPCHECK(hipMalloc(reinterpret_cast<void **>(&partial),
-                           sizeof(float) * (size_t)nc * nparts_max()));
-        HIPCHECK(hipMalloc(reinterpret_cast<void **>(&cols), sizeof(int) * nc));
-        HIPCHECK(hipMalloc(reinterpret_cast<void **>(&nact), sizeof(int)));
-        HIPCHECK(hipHostMalloc(reinterpret_cast<void **>(&h_nact), sizeof(int), hipHostMallocDefault));
+                           sizeof(float) * (size_t)ncap * nparts_max()));
+        HIPCHECK(hipMalloc(reinterpret_cast<void **>(&tiles), sizeof(int) * (ncap / kC)));
+        HIPCHECK(hipMalloc(reinterpret_cast<void **>(&active), sizeof(int) * ncap));
+        HIPCHECK(hipMalloc(reinterpret_cast<void **>(&ntact), sizeof(int)));
+        HIPCHECK(hipHostMalloc(reinterpret_cast<void **>(&h_ntact), sizeof(int),
+                               hipHostMallocDefault));
     }
     int64_t nparts_max() const { return (n + kWG - 1) / kWG; }
     void release()
@@ -409,9 +460,10 @@ struct PCG32 {
         if (!x) return;
         ctx().sync();
         for (float *q : {x, r, p, w, rho, rho_old, init_res, alpha, partial}) HIPCHECK(hipFree(q));
-        HIPCHECK(hipFree(cols));
-        HIPCHECK(hipFree(nact));
-        HIPCHECK(hipHostFree(h_nact));
+        HIPCHECK(hipFree(tiles));
+        HIPCHECK(hipFree(active));
+        HIPCHECK(hipFree(ntact));
+        HIPCHECK(hipHostFree(h_ntact));
         x = nullptr;
     }
 };
@@ -421,21 +473,6 @@ unsigned blocks_for(int64_t n, int cap)
     int64_t b = (n + kWG * 4 - 1) / (kWG * 4);
     if (b > cap) b = cap;
     return (unsigned)(b < 1 ? 1 : b);
-}
-
-void spmm_pw_launch(const SellMat &A, const SellMat &Bm, float sigma, const int *cols,
-                    const int *nact, const float *P, float *Wv, int64_t ld, float *partial,
-                    int with_dot, hipStream_t s)
-{
-    const unsigned g = (unsigned)((A.nrows + kWG - 1) / kWG);
-    if (A.max_width <= 8)
-        k_spmm_pw<8><<<g, kWG, 0, s>>>(A.nrows, A.slice_ptr, A.col, A.val32, Bm.col, Bm.val32,
-                                       sigma, cols, nact, P, Wv, ld, partial, with_dot);
-    else if (A.max_width <= 32)
-        k_spmm_pw<32><<<g, kWG, 0, s>>>(A.nrows, A.slice_ptr, A.col, A.val32, Bm.col, Bm.val32,
-                                        sigma, cols, nact, P, Wv, ld, partial, with_dot);
-    else
-        fail("BlockPCG32: rows wider than 32 entries are not supported", __FILE__, __LINE__);
 }
 
 void pcg32_solve(void *matA, void **mv_b, void **mv_x, int *start, int *end, OPS *ops)
@@ -448,80 +485,85 @@ void pcg32_solve(void *matA, void **mv_b, void **mv_x, int *start, int *end, OPS
     if (nc <= 0) return;
     if (nc > 1024) fail("BlockPCG32: more than 1024 columns", __FILE__, __LINE__);
     if (A->sell.val32 == nullptr) fail("BlockPCG32: A has no FP32 copy", __FILE__, __LINE__);
+    if (A->sell.max_width > 32)
+        fail("BlockPCG32: rows wider than 32 entries are not supported", __FILE__, __LINE__);
     const SellMat *Bs = &A->sell;
     float sigma = 0.f;
     if (S.B != nullptr && S.sigma != 0.0) {
         Mat *Bm = mat_of(S.B, "BlockPCG32 B");
         if (Bm->sell.val32 == nullptr) fail("BlockPCG32: B has no FP32 copy", __FILE__, __LINE__);
-        if (Bm->sell.nslices != A->sell.nslices)
-            fail("BlockPCG32: A and B differ in size", __FILE__, __LINE__);
+        // the shifted kernels walk A's slice pointers for both matrices
+        if (Bm->sell.structure_hash != A->sell.structure_hash)
+            fail("BlockPCG32: a shift needs A and B with the same SELL row structure", __FILE__,
+                 __LINE__);
         Bs = &Bm->sell;
         sigma = (float)S.sigma;
     }
-    // the shifted kernels walk A's slice pointers for both matrices
-    if (Bs != &A->sell && Bs->structure_hash != A->sell.structure_hash)
-        fail("BlockPCG32: a shift needs A and B with the same SELL row structure", __FILE__,
-             __LINE__);
     const int64_t n = A->n;
     S.ensure(n, nc);
     Ctx &c = ctx();
     hipStream_t st = c.stream;
-    const dim3 gcol((unsigned)((n + kWG - 1) / kWG), nc);
-    // refinement: r = fl32(b - op(x)) in FP64, solve op(d) = r from d = 0
-    k_zero32<<<gcol, kWG, 0, st>>>(n, S.x, n);
+    const int ntile = (nc + kC - 1) / kC;
     const unsigned nblk_row = (unsigned)((n + kWG - 1) / kWG);
+    k_zero_tiles<<<dim3(nblk_row, ntile), kWG, 0, st>>>(n, S.x);
     {
         const double *b64 = Bv->d + (int64_t)start[0] * Bv->ldd;
         const double *x64 = Xv->d + (int64_t)start[1] * Xv->ldd;
         const double *vB = (Bs != &A->sell) ? Bs->val : nullptr;
-        if (A->sell.max_width <= 8)
-            k_resid64<8><<<nblk_row, kWG, 0, st>>>(n, A->sell.slice_ptr, A->sell.col, A->sell.val,
-                                                   Bs->col, vB, S.sigma, nc, b64, Bv->ldd, x64,
-                                                   Xv->ldd, S.r, n, S.partial);
-        else if (A->sell.max_width <= 32)
-            k_resid64<32><<<nblk_row, kWG, 0, st>>>(n, A->sell.slice_ptr, A->sell.col,
-                                                    A->sell.val, Bs->col, vB, S.sigma, nc, b64,
-                                                    Bv->ldd, x64, Xv->ldd, S.r, n, S.partial);
-        else
-            fail("BlockPCG32: rows wider than 32 entries are not supported", __FILE__, __LINE__);
+#define RESID(W, SH)                                                                          \
+    k_resid64<W, SH><<<nblk_row, kWG, 0, st>>>(n, A->sell.slice_ptr, A->sell.col, A->sell.val,   \
+                                               Bs->col, vB, S.sigma, nc, b64, Bv->ldd, x64,      \
+                                               Xv->ldd, S.r, S.partial)
+        const bool sh = vB != nullptr;
+        if (A->sell.max_width <= 8) {
+            if (sh) RESID(8, true); else RESID(8, false);
+        } else {
+            if (sh) RESID(32, true); else RESID(32, false);
+        }
+#undef RESID
     }
-    {
-        std::vector<int> all((size_t)nc);
-        for (int k = 0; k < nc; ++k) all[k] = k;
-        HIPCHECK(hipMemcpyAsync(S.cols, all.data(), sizeof(int) * nc, hipMemcpyHostToDevice, st));
-    }
-    const unsigned nb = blocks_for(n, 1024);
-    k_finish_rr<<<1, 1024, 0, st>>>(nc, 0, (int)nblk_row, S.partial, S.cols, S.nact, S.rho,
-                                    S.rho_old, S.init_res, (float)S.rate, (float)S.tol);
-    HIPCHECK(hipMemcpyAsync(S.h_nact, S.nact, sizeof(int), hipMemcpyDeviceToHost, st));
+    k_finish_rr<<<1, 1024, 0, st>>>(nc, 0, (int)nblk_row, S.partial, S.tiles, S.ntact, S.active,
+                                    S.rho, S.rho_old, S.init_res, (float)S.rate, (float)S.tol);
+    HIPCHECK(hipMemcpyAsync(S.h_ntact, S.ntact, sizeof(int), hipMemcpyDeviceToHost, st));
     c.sync();
-    int nact = *S.h_nact;
-    const int nparts_spmm = (int)((n + kWG - 1) / kWG);
-    // algorithmic bytes of one shifted SpMM over k columns: the CCS arrays
-    // of A (and of B when shifted) once, p read once, w written once
+    int ntact = *S.h_ntact;
+    const unsigned nb = blocks_for(n, 1024);
+    // algorithmic bytes of one shifted SpMM over the active tiles: the CCS
+    // arrays of A (and of B when shifted) once, p read once, w written once
     const double mat_bytes = (double)A->sell.nnz_true * 8.0 + 4.0 * (double)(n + 1) +
                              (sigma != 0.f ? (double)Bs->nnz_true * 8.0 : 0.0);
     int it = 0;
-    while (it < S.max_iter && nact > 0) {
-        k_update_p<<<dim3(nb, nact), kWG, 0, st>>>(n, S.cols, S.nact, S.rho, S.rho_old,
-                                                   it == 0 ? 1 : 0, S.r, S.p, n);
-        hipEvent_t ev = c.prof.begin(PROF_SPMM32, mat_bytes + 2.0 * 4.0 * (double)nact * n,
-                                     2.0 * (double)A->sell.nnz_true * nact, st);
-        spmm_pw_launch(A->sell, *Bs, sigma, S.cols, S.nact, S.p, S.w, n, S.partial, 1, st);
+    while (it < S.max_iter && ntact > 0) {
+        k_update_p<<<dim3(nb, ntact), kWG, 0, st>>>(n, S.tiles, S.ntact, S.rho, S.rho_old,
+                                                    it == 0 ? 1 : 0, S.r, S.p);
+        hipEvent_t ev = c.prof.begin(PROF_SPMM32,
+                                     mat_bytes + 2.0 * 4.0 * kC * (double)ntact * n,
+                                     2.0 * (double)A->sell.nnz_true * kC * ntact, st);
+        if (A->sell.max_width <= 8)
+            k_spmm_pw<8><<<nblk_row, kWG, 0, st>>>(n, A->sell.slice_ptr, A->sell.col,
+                                                   A->sell.val32, Bs->col, Bs->val32, sigma,
+                                                   S.tiles, S.ntact, S.p, S.w, S.partial);
+        else
+            k_spmm_pw<32><<<nblk_row, kWG, 0, st>>>(n, A->sell.slice_ptr, A->sell.col,
+                                                    A->sell.val32, Bs->col, Bs->val32, sigma,
+                                                    S.tiles, S.ntact, S.p, S.w, S.partial);
         c.prof.end(ev, st);
-        k_finish_pw<<<nact, 64, 0, st>>>(S.cols, S.nact, nparts_spmm, S.partial, S.rho, S.alpha);
-        k_update_xr<<<dim3(nb, nact), kWG, 0, st>>>(n, S.cols, S.nact, S.alpha, S.p, S.w, S.x,
-                                                    S.r, n, S.partial);
-        k_finish_rr<<<1, 1024, 0, st>>>(nc, 1, (int)nb, S.partial, S.cols, S.nact, S.rho,
-                                        S.rho_old, S.init_res, (float)S.rate, (float)S.tol);
-        HIPCHECK(hipMemcpyAsync(S.h_nact, S.nact, sizeof(int), hipMemcpyDeviceToHost, st));
+        k_finish_pw<<<ntact * kC, 64, 0, st>>>(S.tiles, S.ntact, (int)nblk_row, S.partial, S.rho,
+                                               S.active, S.alpha);
+        k_update_xr<<<dim3(nb, ntact), kWG, 0, st>>>(n, S.tiles, S.ntact, S.alpha, S.p, S.w,
+                                                     S.x, S.r, S.partial);
+        k_finish_rr<<<1, 1024, 0, st>>>(nc, 1, (int)nb, S.partial, S.tiles, S.ntact, S.active,
+                                        S.rho, S.rho_old, S.init_res, (float)S.rate,
+                                        (float)S.tol);
+        HIPCHECK(hipMemcpyAsync(S.h_ntact, S.ntact, sizeof(int), hipMemcpyDeviceToHost, st));
         c.sync();
-        nact = *S.h_nact;
+        ntact = *S.h_ntact;
         ++it;
     }
     S.iters += it;
-    // x += d in FP64 (and d back to zero)
-    k_add_up<<<gcol, kWG, 0, st>>>(n, S.x, n, Xv->d + (int64_t)start[1] * Xv->ldd, Xv->ldd);
+    // x += d in FP64
+    k_add_up<<<dim3(nblk_row, nc), kWG, 0, st>>>(n, nc, S.x, Xv->d + (int64_t)start[1] * Xv->ldd,
+                                                 Xv->ldd);
 }
 
 }  // namespace

@@ -104,7 +104,7 @@ def solve(A, B, nev: int, block_size: int, nev_init: int, nev_max: int,
           tol=(1e-1, 1e-8), gap_min: float = 1e-5, shift: int = 1,
           max_iter: int = 1000, mode: str = "fp64", rand: str = "libc",
           seed: int = 0, eig: str = "auto", want_vectors: bool = False,
-          print_level: int = 0, mats=None) -> SolveResult:
+          print_level: int = 0, mats=None, orth: str | None = None) -> SolveResult:
     """Smallest `nev` eigenpairs of A x = lambda B x on the GPU.
 
     mode "fp64": the reference-equivalent path, every operation in FP64.
@@ -113,6 +113,8 @@ def solve(A, B, nev: int, block_size: int, nev_init: int, nev_max: int,
     solver, src/ops_eig_sol_gcg.c:503).
     rand "libc": initial block from srand(seed)/rand() as the reference;
     "device": counter-based generator on the device.
+    orth "mgs" (the reference's orthogonalization) or "cholqr" (block
+    Cholesky QR); default "mgs" for fp64 and "cholqr" for mixed.
     `mats` may pass pre-uploaded (DeviceMatrix A, DeviceMatrix B).
     """
     lib = abi.load()
@@ -146,6 +148,10 @@ def solve(A, B, nev: int, block_size: int, nev_init: int, nev_max: int,
     if eig != "auto":
         lib.HIPGCG_SetDenseEigensolver(ops, abi.GCGE_EIG_HOST if eig == "host"
                                        else abi.GCGE_EIG_DEVICE)
+    if orth is None:
+        orth = "cholqr" if mode == "mixed" else "mgs"
+    lib.HIPGCG_SetOrthMethod(ops, abi.GCGE_ORTH_CHOLQR if orth == "cholqr"
+                             else abi.GCGE_ORTH_MGS)
     if mode == "mixed":
         lib.HIPGCG_MultiLinearSolverSetup_BlockPCG32(30, 1e-2, 1e-14, hB, 0.0, ops)
     ev = np.zeros(nev_max)
@@ -175,7 +181,7 @@ def solve_problem(prob, **kw) -> SolveResult:
 
 def solve_host_dense(A: np.ndarray, B: np.ndarray, nev: int, block_size: int, nev_init: int,
                      nev_max: int, tol=(1e-1, 1e-8), gap_min: float = 1e-5, shift: int = 1,
-                     max_iter: int = 1000, seed: int = 0) -> SolveResult:
+                     max_iter: int = 1000, seed: int = 0, orth: str = "mgs") -> SolveResult:
     """This library's driver over its host dense table (no device use).
 
     Dense A, B in the LAPACKMAT layout; runs the same control flow as the
@@ -202,6 +208,8 @@ def solve_host_dense(A: np.ndarray, B: np.ndarray, nev: int, block_size: int, ne
                                 0, 0, mv_ws, None, None, ops)
     lib.HIPGCG_Setparameters(gap_min, shift, ops)
     lib.HIPGCG_SetDenseEigensolver(ops, abi.GCGE_EIG_HOST)
+    lib.HIPGCG_SetOrthMethod(ops, abi.GCGE_ORTH_CHOLQR if orth == "cholqr"
+                             else abi.GCGE_ORTH_MGS)
     ev = np.zeros(nev_max)
     nconv = C.c_int(nev)
     t0 = time.perf_counter()

@@ -164,6 +164,18 @@ void HIPGCG_GetStats(OPS *ops, int64_t *stats, double *times);
 #define GCGE_EIG_DEVICE 1
 void HIPGCG_SetDenseEigensolver(OPS *ops, int which);
 
+/* B-orthonormalization of the W block and of the initial block:
+ * GCGE_ORTH_MGS (default) is the reference's block modified Gram-Schmidt
+ * with column-by-column self-orthonormalization (src/ops_orth.c:203-394);
+ * GCGE_ORTH_CHOLQR keeps its removal of the earlier columns but
+ * orthonormalizes the block itself by Cholesky QR applied twice, falling
+ * back to MGS when the block is numerically rank deficient.  Same
+ * subspace, different rounding: results agree in eigenvalues and
+ * residuals, not bit for bit. */
+#define GCGE_ORTH_MGS 0
+#define GCGE_ORTH_CHOLQR 1
+void HIPGCG_SetOrthMethod(OPS *ops, int which);
+
 /* ------------------------------------------------------------------ */
 /* Mixed precision: FP32 BlockPCG in the MultiLinearSolver slot        */
 /* ------------------------------------------------------------------ */

@@ -69,10 +69,19 @@ def test_mixed_cfg1(hip):
     check_pairs(prob, res, 2, 128)
 
 
-def test_mixed_3d(hip):
+@pytest.mark.parametrize("orth", ["cholqr", "mgs"])
+def test_mixed_3d(hip, orth):
     prob = P.stencil_problem(16, 3, 40)
-    res = S.solve_problem(prob, shift=0, mode="mixed", want_vectors=True)
+    res = S.solve_problem(prob, shift=0, mode="mixed", want_vectors=True, orth=orth)
     check_pairs(prob, res, 3, 16)
+
+
+def test_fp64_cholqr(hip):
+    """The FP64 path with the block Cholesky QR orthogonalization."""
+    prob = P.stencil_problem(12, 3, 30)
+    g = golden("3d_12", 1)
+    res = S.solve_problem(prob, shift=1, want_vectors=True, orth="cholqr")
+    check_pairs(prob, res, 3, 12, g["num_iter"], 10)
 
 
 def test_device_random_and_eig(hip):

@@ -41,6 +41,25 @@ def test_host_driver_matches_reference(hip, name, n, dim, nev, shift):
     assert rr.max() <= 1.01e-8
 
 
+@pytest.mark.parametrize("name,n,dim,nev,shift", [("2d_32", 32, 2, 10, 1),
+                                                  ("2d_24", 24, 2, 20, 1)])
+def test_host_driver_cholqr(hip, name, n, dim, nev, shift):
+    """Block Cholesky QR in place of the column MGS: the same pairs as the
+    reference (a different rounding path, so the iteration count may move
+    by a few)."""
+    prob = P.stencil_problem(n, dim, nev)
+    g = golden(name, shift, "lapack")
+    r = S.solve_host_dense(prob.A.toarray(), prob.B.toarray(), nev, prob.block_size,
+                           prob.nev_init, prob.nev_max, shift=shift, orth="cholqr")
+    assert r.nev_conv >= nev
+    assert abs(r.num_iter - g["num_iter"]) <= 5
+    k = min(r.nev_conv, g["nev_conv"])
+    ref = np.array(g["eval"][:k])
+    assert np.max(np.abs(r.eval[:k] - ref) / ref) <= 1e-9
+    rr = O.relative_residuals(prob.A, prob.B, r.eval[:k], r.evec[:, :k])
+    assert rr.max() <= 1.01e-8
+
+
 def test_host_driver_max_iter_reports_nonconvergence(hip):
     """numIterMax = 1: a reported non-convergence, not an error
     (gcge.c:525-531 maps it to EPS_DIVERGED_ITS)."""
