@@ -16,7 +16,11 @@ Two paths are run on the same problem:
 reported beside it.
 
 The roofline object is for the dominant kernel class of the mixed path,
-timed with HIP events on the library's stream over the timed region.
+timed with HIP events on the library's stream over the timed region; its
+`traffic` (measured HBM bytes per launch) is read from the hardware-counter
+summary profiles/pmc_traffic_cfg2_mixed.txt when that file is present.
+`--copy-probe` runs only a column copy of known size (the counter
+calibration kernel) and exits.
 The CPU baseline runs the reference GCGE (compiled from its sources in
 oracle/_ref, OpenMP build) on the same workload for a bounded sample and
 extrapolates with the GPU run's iteration count.
@@ -101,6 +105,75 @@ def run_path(prob, n, dim, mode, steps, warmup, mats, want_check=True):
     return out
 
 
+def copy_probe(reps: int = 5):
+    """A plain device copy of known size through the table (MatDotMultiVec
+    with a NULL matrix): 120 columns of 2,097,152 doubles, read once and
+    written once, 2,013,265,920 bytes each way per launch."""
+    import ctypes as C
+    from gcge_amd import abi, problems as P
+    from gcge_amd.solver import DeviceMatrix, HipTable
+    t = HipTable()
+    lib = t.lib
+    prob = P.stencil_problem(128, 3, 100, 120, 220, 220)
+    A = DeviceMatrix(prob.A)
+    lib.gcge_hip_set_random_mode(abi.GCGE_HIP_RAND_DEVICE, 5)
+    X, Y = t.create_multivec(120, A), t.create_multivec(120, A)
+    t.slot("MultiVecSetRandomValue")(X, 0, 120, t.ops)
+    s, e = (C.c_int * 2)(0, 0), (C.c_int * 2)(120, 120)
+    lib.gcge_hip_synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        t.slot("MatDotMultiVec")(None, X, Y, s, e, t.ops)
+    lib.gcge_hip_synchronize()
+    dt = (time.perf_counter() - t0) / reps
+    nbytes = 120 * prob.n * 8
+    print(json.dumps({"copy_probe": {"bytes_read": nbytes, "bytes_written": nbytes,
+                                     "ms": 1e3 * dt, "GBs": 2 * nbytes / dt / 1e9}}))
+    t.destroy_multivec(X, 120)
+    t.destroy_multivec(Y, 120)
+
+
+def rocprof_summary(d: str):
+    """Condenses rocprofv3 CSV output under directory `d` (any depth) into
+    one line per kernel: calls and average duration from *_kernel_stats.csv,
+    mean counter values per dispatch from *_counter_collection.csv."""
+    import csv
+    import glob
+    for p in sorted(glob.glob(os.path.join(d, "**", "*_kernel_stats.csv"), recursive=True)):
+        print(f"# kernel stats: {os.path.relpath(p, d)}")
+        print(f"{'calls':>8} {'total_ms':>10} {'avg_us':>10} {'pct':>6}  kernel")
+        for r in csv.DictReader(open(p)):
+            print(f"{int(r['Calls']):8d} {float(r['TotalDurationNs']) / 1e6:10.2f} "
+                  f"{float(r['AverageNs']) / 1e3:10.2f} {float(r['Percentage']):6.2f}  "
+                  f"{r['Name'][:110]}")
+    for p in sorted(glob.glob(os.path.join(d, "**", "*_counter_collection.csv"),
+                              recursive=True)):
+        agg = {}
+        for r in csv.DictReader(open(p)):
+            key = (r["Kernel_Name"][:110], r["Counter_Name"])
+            # one row per dispatch and counter
+            a = agg.setdefault(key, [0, 0.0])
+            a[0] += 1
+            a[1] += float(r["Counter_Value"])
+        print(f"# counters per dispatch: {os.path.relpath(p, d)}")
+        print(f"{'dispatches':>10} {'counter':>12} {'mean':>16}  kernel")
+        for (k, c), (cnt, tot) in sorted(agg.items(), key=lambda kv: -kv[1][1]):
+            print(f"{cnt:10d} {c:>12} {tot / cnt:16.1f}  {k}")
+
+
+def measured_traffic(kernel: str):
+    """HBM bytes per launch of `kernel` from the committed counter summary:
+    a line 'traffic_bytes_per_launch <kernel> <bytes>'."""
+    p = os.path.join(ROOT, "profiles", "pmc_traffic_cfg2_mixed.txt")
+    if not os.path.exists(p):
+        return None
+    for ln in open(p):
+        f = ln.split()
+        if len(f) == 3 and f[0] == "traffic_bytes_per_launch" and f[1] == kernel:
+            return float(f[2])
+    return None
+
+
 def cpu_baseline(prob, gpu_iters, nev_conv, iters_sample):
     """Reference GCGE (OpenMP build) on the same workload for a bounded
     number of outer iterations, in a child process; extrapolated."""
@@ -136,17 +209,25 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1)
-    ap.add_argument("--warmup", type=int, default=0)
+    ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", default="cfg2")
     ap.add_argument("--modes", default="fp64,mixed")
     ap.add_argument("--cpu-baseline", default="on", choices=("on", "off"))
     ap.add_argument("--cpu-iters", type=int, default=1)
+    ap.add_argument("--copy-probe", action="store_true")
+    ap.add_argument("--rocprof-summary", metavar="DIR")
     args = ap.parse_args()
+    if args.rocprof_summary:
+        rocprof_summary(args.rocprof_summary)
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1",
                           "n_gpus": args.gpus}))
         sys.exit(2)
 
+    if args.copy_probe:
+        copy_probe()
+        return
     from gcge_amd import abi, problems as P, solver as S
     abi.load().gcge_host_set_print_target(1)  # monitor lines to stderr
     t0 = time.perf_counter()
@@ -156,10 +237,11 @@ def main():
     mats = (S.DeviceMatrix(prob.A, fp32=True), S.DeviceMatrix(prob.B, fp32=True))
     t_upload = time.perf_counter() - t0
     log(f"upload {t_upload:.2f}s")
-    # load the kernels and the dense eigensolver once (not a step)
-    pp = P.stencil_problem(12, 3, 30)
-    S.solve_problem(pp, shift=1, rand="device", eig="device")
-    S.solve_problem(pp, shift=0, mode="mixed", rand="device")
+    if args.warmup == 0:
+        # load the kernels and the dense eigensolver once (not a step)
+        pp = P.stencil_problem(12, 3, 30)
+        S.solve_problem(pp, shift=1, rand="device", eig="device")
+        S.solve_problem(pp, shift=0, mode="mixed", rand="device")
 
     results = {}
     for mode in args.modes.split(","):
@@ -175,16 +257,19 @@ def main():
     roof = None
     if dom:
         kd = kern[dom]
-        bound_mfma = dom in ("gram", "lincomb") and kd["flops"] / max(kd["bytes"], 1) > 9.8
-        if bound_mfma:
-            roof = {"bound": "mfma", "kernel": dom, "achieved": kd["TFs"],
-                    "peak": FP64_PEAK_TFS, "unit": "TFLOP/s",
-                    "frac": kd["TFs"] / FP64_PEAK_TFS, "traffic": None}
+        # FP64 ridge point: 78.6 TFLOP/s / 8 TB/s = 9.8 flop/byte
+        intensity = kd["flops"] / max(kd["bytes"], 1.0)
+        common = {"kernel": dom, "avg_launch_us": kd["avg_us"], "launches": kd["launches"],
+                  "algorithmic_bytes_per_launch": kd["bytes"] / kd["launches"],
+                  "algorithmic_flops_per_launch": kd["flops"] / kd["launches"],
+                  "flop_per_byte": intensity,
+                  "traffic": measured_traffic(dom)}
+        if intensity > FP64_PEAK_TFS / (HBM_PEAK_GBS / 1e3):
+            roof = {"bound": "mfma", "achieved": kd["TFs"], "peak": FP64_PEAK_TFS,
+                    "unit": "TFLOP/s", "frac": kd["TFs"] / FP64_PEAK_TFS, **common}
         else:
-            roof = {"bound": "hbm", "kernel": dom, "achieved": kd["GBs"],
-                    "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": kd["GBs"] / HBM_PEAK_GBS,
-                    "traffic": None, "avg_launch_us": kd["avg_us"],
-                    "algorithmic_bytes_per_launch": kd["bytes"] / kd["launches"]}
+            roof = {"bound": "hbm", "achieved": kd["GBs"], "peak": HBM_PEAK_GBS,
+                    "unit": "GB/s", "frac": kd["GBs"] / HBM_PEAK_GBS, **common}
     cpu, why = (None, "disabled")
     if args.cpu_baseline == "on" and args.config == "cfg2":
         cpu, why = cpu_baseline(prob, hr["num_iter"], hr["nev_conv"], args.cpu_iters)

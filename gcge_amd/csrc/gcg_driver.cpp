@@ -40,6 +40,8 @@ struct Solver {
     // algorithm parameters (static defaults of src/ops_eig_sol_gcg.c:1573-1603)
     int check_conv_max_num = 15;
     OrthParams orthX, orthP, orthW;
+    // block orthonormalizations by CholQR2 and how many fell back to MGS
+    int64_t cholqr_blocks = 0, cholqr_fallbacks = 0;
     int cg_max_iter = 40;
     double cg_rate = 1e-2, cg_tol = 1e-8;
     std::string cg_tol_type = "abs";
@@ -185,22 +187,37 @@ bool cholesky_upper(int n, const std::vector<double> &G, std::vector<double> &R,
 
 // Block orthonormalization of x[:, start_x:*end_x] in the B inner product:
 // the previous columns are removed as in MGS, then Cholesky QR is applied
-// twice to the whole block (CholQR2): G = X^T B X = R^T R, X <- X R^{-1}.
-// The same subspace as the reference's column-by-column MGS with a few
-// block products instead of one B-product and two passes per column.
-// When G is not numerically positive definite (dependent columns) the
-// block falls back to the column-by-column MGS, which drops them.
+// twice to the whole block (CholQR2): G = X^T B X, scaled to unit diagonal
+// G' = D^{-1} G D^{-1} (D = diag(G)^{1/2}, so columns of very different
+// norms -- the corrections of nearly converged pairs are tiny -- do not
+// look dependent), G' = R^T R, X <- X D^{-1} R^{-1}.  The same subspace as
+// the reference's column-by-column MGS with a few block products instead
+// of one B-product and two passes per column.  A column whose B-norm is
+// below zero_tol (which MGS drops) or a numerically dependent block sends
+// the block to the column-by-column MGS.
 void cholqr(void **x, int start_x, int *end_x, void *B, const OrthParams &p, void **mv_ws,
-            OPS *ops)
+            OPS *ops, int64_t *fallbacks)
 {
     if (*end_x <= start_x) return;
     remove_previous(x, start_x, *end_x, B, p, mv_ws, ops);
     const int n = *end_x - start_x;
-    std::vector<double> G((size_t)n * n), R, Rinv((size_t)n * n);
+    std::vector<double> G((size_t)n * n), R, Rinv((size_t)n * n), dinv((size_t)n);
     for (int pass = 0; pass < 2; ++pass) {
         int s[2] = {start_x, start_x}, e[2] = {*end_x, *end_x};
         ops->MultiVecQtAP('S', 'S', x, B, x, 0, s, e, G.data(), n, mv_ws, ops);
-        if (!cholesky_upper(n, G, R, 1e-12)) {
+        bool ok = true;
+        for (int i = 0; i < n && ok; ++i) {
+            const double d = std::sqrt(std::max(G[(size_t)n * i + i], 0.0));
+            ok = d >= p.zero_tol;
+            dinv[i] = ok ? 1.0 / d : 0.0;
+        }
+        if (ok) {
+            for (int j = 0; j < n; ++j)
+                for (int i = 0; i < n; ++i) G[(size_t)n * j + i] *= dinv[i] * dinv[j];
+            ok = cholesky_upper(n, G, R, 1e-10);
+        }
+        if (!ok) {
+            *fallbacks += 1;
             mgs_blocks(x, start_x, end_x, B, p, mv_ws, ops);
             return;
         }
@@ -214,7 +231,9 @@ void cholqr(void **x, int start_x, int *end_x, void *B, const OrthParams &p, voi
                 Rinv[(size_t)n * j + i] = -t / R[(size_t)n * i + i];
             }
         }
-        // X <- X R^{-1} through the workspace (the product cannot run in place)
+        for (int j = 0; j < n; ++j)
+            for (int i = 0; i <= j; ++i) Rinv[(size_t)n * j + i] *= dinv[i];
+        // X <- X D^{-1} R^{-1} through the workspace (the product cannot run in place)
         int s1[2] = {start_x, 0}, e1[2] = {*end_x, n};
         ops->MultiVecLinearComb(x, mv_ws, 0, s1, e1, Rinv.data(), n, nullptr, 0, ops);
         int s2[2] = {0, start_x}, e2[2] = {n, *end_x};
@@ -425,7 +444,10 @@ int nV(const Solver &S) { return S.sizeV - S.sizeC; }
 void orthonormalize(Solver &S, void **V, int start, int *end, const OrthParams &p, void **ws)
 {
     if (S.orth_method == GCGE_ORTH_CHOLQR)
-        cholqr(V, start, end, S.B, p, ws, S.ops);
+    {
+        S.cholqr_blocks += 1;
+        cholqr(V, start, end, S.B, p, ws, S.ops, &S.cholqr_fallbacks);
+    }
     else
         mgs(V, start, end, S.B, p, ws, S.ops);
 }
@@ -706,6 +728,8 @@ void gcg(void *A, void *B, double *eval, void **evec, int nevGiven, int *nevConv
     S.B = B;
     for (double &t : S.times) t = 0.0;
     S.bpcg_iters = 0;
+    S.cholqr_blocks = 0;
+    S.cholqr_fallbacks = 0;
     S.linsolves = 0;
     const int bs = S.block_size, nevInit = S.nevInit, nevMax = S.nevMax;
     if (!(nevInit >= nevGiven && nevInit <= nevMax && (nevInit >= 3 * bs || nevInit == nevMax) &&
@@ -935,5 +959,12 @@ void HIPGCG_GetStats(OPS *ops, int64_t *stats, double *times)
 void HIPGCG_SetDenseEigensolver(OPS *ops, int which) { solver_of(ops)->eig_backend = which; }
 
 void HIPGCG_SetOrthMethod(OPS *ops, int which) { solver_of(ops)->orth_method = which; }
+
+void HIPGCG_GetOrthStats(OPS *ops, int64_t *cholqr_blocks, int64_t *mgs_fallbacks)
+{
+    const Solver *S = solver_of(ops);
+    *cholqr_blocks = S->cholqr_blocks;
+    *mgs_fallbacks = S->cholqr_fallbacks;
+}
 
 }  // extern "C"

@@ -31,6 +31,8 @@ class SolveResult:
     linsolves: int = 0
     phases: dict = field(default_factory=dict)
     evec: np.ndarray | None = None
+    cholqr_blocks: int = 0
+    cholqr_fallbacks: int = 0
 
 
 class DeviceMatrix:
@@ -165,12 +167,14 @@ def solve(A, B, nev: int, block_size: int, nev_init: int, nev_max: int,
     stats = (C.c_int64 * 4)()
     times = (C.c_double * 10)()
     lib.HIPGCG_GetStats(ops, stats, times)
+    cq = (C.c_int64(), C.c_int64())
+    lib.HIPGCG_GetOrthStats(ops, C.byref(cq[0]), C.byref(cq[1]))
     vecs = t.download(evec, 0, nev_max, n) if want_vectors else None
     lib.HIPGCG_Destroy(mv_ws, None, None, nev_max, block_size, ops)
     t.destroy_multivec(evec, nev_max)
     lib.HIPGCG_Free(ops)
     return SolveResult(ev, nconv.value, int(stats[0]), secs, int(stats[2]), int(stats[3]),
-                       dict(zip(PHASES, list(times))), vecs)
+                       dict(zip(PHASES, list(times))), vecs, cq[0].value, cq[1].value)
 
 
 def solve_problem(prob, **kw) -> SolveResult:
@@ -219,6 +223,8 @@ def solve_host_dense(A: np.ndarray, B: np.ndarray, nev: int, block_size: int, ne
     stats = (C.c_int64 * 4)()
     times = (C.c_double * 10)()
     lib.HIPGCG_GetStats(ops, stats, times)
+    cq = (C.c_int64(), C.c_int64())
+    lib.HIPGCG_GetOrthStats(ops, C.byref(cq[0]), C.byref(cq[1]))
     ev_mv = C.cast(evec, C.POINTER(abi.DenseMV)).contents
     vecs = np.ctypeslib.as_array(ev_mv.data, shape=(nev_max * n,)).reshape(
         (n, nev_max), order="F").copy()
@@ -226,4 +232,4 @@ def solve_host_dense(A: np.ndarray, B: np.ndarray, nev: int, block_size: int, ne
     abi.slot(ops, "MultiVecDestroy")(C.byref(evec), nev_max, ops)
     lib.HIPGCG_Free(ops)
     return SolveResult(ev, nconv.value, int(stats[0]), secs, int(stats[2]), int(stats[3]),
-                       dict(zip(PHASES, list(times))), vecs)
+                       dict(zip(PHASES, list(times))), vecs, cq[0].value, cq[1].value)

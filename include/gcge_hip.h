@@ -158,8 +158,9 @@ void HIPGCG_Free(OPS *ops);
 void HIPGCG_GetStats(OPS *ops, int64_t *stats, double *times);
 
 /* Dense symmetric eigensolver used by the Rayleigh-Ritz step:
- * GCGE_EIG_HOST (own Householder + implicit QL on the host) or
- * GCGE_EIG_DEVICE (rocSOLVER dsyevd on the device). */
+ * GCGE_EIG_HOST (cyclic Jacobi on the host) or GCGE_EIG_DEVICE
+ * (rocSOLVER dsyevd on the device).  Without a call, the device solver
+ * is used for projected matrices of order 96 and above. */
 #define GCGE_EIG_HOST 0
 #define GCGE_EIG_DEVICE 1
 void HIPGCG_SetDenseEigensolver(OPS *ops, int which);
@@ -175,6 +176,9 @@ void HIPGCG_SetDenseEigensolver(OPS *ops, int which);
 #define GCGE_ORTH_MGS 0
 #define GCGE_ORTH_CHOLQR 1
 void HIPGCG_SetOrthMethod(OPS *ops, int which);
+/* Counts of the last solve: blocks orthonormalized with GCGE_ORTH_CHOLQR,
+ * and how many of them fell back to MGS. */
+void HIPGCG_GetOrthStats(OPS *ops, int64_t *cholqr_blocks, int64_t *mgs_fallbacks);
 
 /* ------------------------------------------------------------------ */
 /* Mixed precision: FP32 BlockPCG in the MultiLinearSolver slot        */

@@ -74,6 +74,8 @@ def test_mixed_3d(hip, orth):
     prob = P.stencil_problem(16, 3, 40)
     res = S.solve_problem(prob, shift=0, mode="mixed", want_vectors=True, orth=orth)
     check_pairs(prob, res, 3, 16)
+    if orth == "cholqr":
+        assert res.cholqr_blocks >= res.num_iter and res.cholqr_fallbacks == 0
 
 
 def test_fp64_cholqr(hip):

@@ -52,6 +52,9 @@ def test_host_driver_cholqr(hip, name, n, dim, nev, shift):
     r = S.solve_host_dense(prob.A.toarray(), prob.B.toarray(), nev, prob.block_size,
                            prob.nev_init, prob.nev_max, shift=shift, orth="cholqr")
     assert r.nev_conv >= nev
+    # every W block went through CholQR2 (unit-diagonal scaling keeps the
+    # tiny corrections of nearly converged pairs from looking dependent)
+    assert r.cholqr_blocks >= r.num_iter and r.cholqr_fallbacks == 0
     assert abs(r.num_iter - g["num_iter"]) <= 5
     k = min(r.nev_conv, g["nev_conv"])
     ref = np.array(g["eval"][:k])
