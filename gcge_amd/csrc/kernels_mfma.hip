@@ -173,18 +173,24 @@ GramPlanM plan(int64_t nrows, int m, int n)
 // ---------------------------------------------------------------------
 // LinearComb: Y (nrows x n) = X (nrows x m) C (m x n) + Y diag(beta),
 // evaluated as Y^T = C^T X^T.  A wave owns 64 rows x 64 columns of Y
-// (4 x 4 MFMA tiles): per step of 4 k it loads four X operands (lane l
+// (4 x 4 MFMA tiles): per step of 4 k it takes four X operands (lane l
 // reads X[i + l%16, k + l/16], 128-byte row segments straight from HBM)
 // and four C^T operands (from L1/L2: C is small) and issues 16 MFMAs, so
-// each operand feeds four matrix-core instructions.  The accumulator
-// holds 16 consecutive rows of one column per lane group, so the stores
-// are contiguous.  Ct is C^T packed k-major: Ct[k * ldct + j].  A
-// workgroup (4 waves) covers 256 rows; consecutive workgroups on one XCD
+// each operand feeds four matrix-core instructions.  Operands sit in a
+// ring of kRing one-step buffers: the loads for step s + kRing - 1 are
+// issued before the MFMAs of step s, and at 224 registers two waves share
+// each SIMD, so HBM latency hides behind the other wave's MFMAs.  The
+// accumulator holds 16 consecutive rows of one column per lane group, so
+// the stores are contiguous.  Ct is C^T packed k-major: Ct[k * ldct + j].
+// A workgroup (4 waves) covers 256 rows; consecutive workgroups on one XCD
 // take the column tiles of the same rows.
 // ---------------------------------------------------------------------
+constexpr int kRing = 4;
+
+template <int BM>
 __global__ __launch_bounds__(kWG) void k_lincomb_mfma(
     int64_t nrows, int m, int n, const double *__restrict__ X, int64_t ldx,
-    const double *__restrict__ Ct, int ldct, int beta_mode, double beta_scalar,
+    const double *__restrict__ Ct, int ldct, double beta_scalar,
     const double *__restrict__ beta, double *__restrict__ Y, int64_t ldy, int ntj)
 {
     const int64_t lb = xcd_swizzle(blockIdx.x, gridDim.x);
@@ -205,45 +211,66 @@ __global__ __launch_bounds__(kWG) void k_lincomb_mfma(
     bool jok[4];
 #pragma unroll
     for (int q = 0; q < 4; ++q) jok[q] = (j0 + q * 16 + li) < n;
+    // BM (beta mode) 0: overwrite; 1: scalar beta; 2: per-column beta.  The
+    // accumulators start from Y diag(beta), read 16 at a time (a scheduling
+    // barrier after each row tile keeps the 64 loads from being in flight at
+    // once, which would double the registers and halve the occupancy)
     d4 acc[4][4];
 #pragma unroll
-    for (int a = 0; a < 4; ++a)
+    for (int a = 0; a < 4; ++a) {
+        // out-of-range rows and columns read a clamped in-range element and
+        // are zeroed by the select (no branch per element)
+        const int64_t rowc = rok[a] ? i0 + a * 16 + li : nrows - 1;
 #pragma unroll
-        for (int q = 0; q < 4; ++q) acc[a][q] = d4{0.0, 0.0, 0.0, 0.0};
-    // k runs in chunks of 16 (four MFMA steps); the next chunk's operands
-    // are loaded while this chunk's 64 MFMAs run, which covers HBM latency
-    // at one wave per SIMD
-    double bx[4][4], ac[4][4], nbx[4][4], nac[4][4];
-    auto load_chunk = [&](int k0) {
+        for (int q = 0; q < 4; ++q) {
+            d4 init;
 #pragma unroll
-        for (int s = 0; s < 4; ++s) {
-            const int k = k0 + 4 * s + lk;
-            const bool kok = k < m;
-#pragma unroll
-            for (int a = 0; a < 4; ++a)
-                nbx[s][a] = (kok && rok[a]) ? xr[a][(int64_t)k * ldx] : 0.0;
-            const double *cr = Ct + (int64_t)(kok ? k : 0) * ldct + j0 + li;
-#pragma unroll
-            for (int q = 0; q < 4; ++q) nac[s][q] = (kok && jok[q]) ? cr[q * 16] : 0.0;
-        }
-    };
-    load_chunk(0);
-    for (int k0 = 0; k0 < m; k0 += 16) {
-#pragma unroll
-        for (int s = 0; s < 4; ++s)
-#pragma unroll
-            for (int a = 0; a < 4; ++a) {
-                bx[s][a] = nbx[s][a];
-                ac[s][a] = nac[s][a];
+            for (int v = 0; v < 4; ++v) {
+                const int j = j0 + q * 16 + lk + 4 * v;
+                if (BM == 0) {
+                    init[v] = 0.0;
+                } else {
+                    const int jc = j < n ? j : n - 1;
+                    const double bb = BM == 2 ? beta[jc] : beta_scalar;
+                    const double y = Y[rowc + (int64_t)jc * ldy];
+                    init[v] = (rok[a] && j < n && bb != 0.0) ? bb * y : 0.0;
+                }
             }
-        if (k0 + 16 < m) load_chunk(k0 + 16);
+            // D = 0 * 0 + init moves the value into the accumulator registers
+            acc[a][q] = BM == 0 ? init : mfma(0.0, 0.0, init);
+        }
+        if (BM != 0) asm volatile("" ::: "memory");
+    }
+    // bx[b][a]: X operand of row tile a, ac[b][q]: C^T operand of column
+    // tile q, for the step held in ring buffer b; steps past m load zeros
+    double bx[kRing][4], ac[kRing][4];
+    // one step into ring buffer B (a macro: the ring index must stay a
+    // compile-time constant for the buffers to live in registers)
+#define GCGE_LC_LOAD(B, STEP)                                                         \
+    {                                                                                 \
+        const int k_ = (STEP) * 4 + lk;                                               \
+        const bool kok_ = k_ < m;                                                     \
+        _Pragma("unroll") for (int a = 0; a < 4; ++a) bx[B][a] =                      \
+            (kok_ && rok[a]) ? xr[a][(int64_t)k_ * ldx] : 0.0;                        \
+        const double *cr_ = Ct + (int64_t)(kok_ ? k_ : 0) * ldct + j0 + li;           \
+        _Pragma("unroll") for (int q = 0; q < 4; ++q) ac[B][q] =                      \
+            (kok_ && jok[q]) ? cr_[q * 16] : 0.0;                                     \
+    }
 #pragma unroll
-        for (int s = 0; s < 4; ++s)
+    for (int b = 0; b < kRing - 1; ++b) GCGE_LC_LOAD(b, b);
+    const int nsteps = (m + 3) / 4;
+    for (int s0 = 0; s0 < nsteps; s0 += kRing) {
+#pragma unroll
+        for (int u = 0; u < kRing; ++u) {
+            // refill the buffer the previous step used
+            GCGE_LC_LOAD((u + kRing - 1) % kRing, s0 + u + kRing - 1);
 #pragma unroll
             for (int a = 0; a < 4; ++a)
 #pragma unroll
-                for (int q = 0; q < 4; ++q) acc[a][q] = mfma(ac[s][q], bx[s][a], acc[a][q]);
+                for (int q = 0; q < 4; ++q) acc[a][q] = mfma(ac[u][q], bx[u][a], acc[a][q]);
+        }
     }
+#undef GCGE_LC_LOAD
     // lane l holds (Y^T)[j0 + q*16 + l/16 + 4v][i0 + a*16 + l%16]
 #pragma unroll
     for (int a = 0; a < 4; ++a) {
@@ -254,15 +281,7 @@ __global__ __launch_bounds__(kWG) void k_lincomb_mfma(
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
                 const int j = j0 + q * 16 + lk + 4 * v;
-                if (j < n) {
-                    double *y = Y + row + (int64_t)j * ldy;
-                    double val = acc[a][q][v];
-                    if (beta_mode != 0) {
-                        const double bb = beta_mode == 2 ? beta[j] : beta_scalar;
-                        if (bb != 0.0) val += bb * *y;
-                    }
-                    *y = val;
-                }
+                if (j < n) Y[row + (int64_t)j * ldy] = acc[a][q][v];
             }
     }
 }
@@ -296,9 +315,16 @@ void lincomb_mfma(int64_t nrows, int m, int n, const double *X, int64_t ldx, con
     if (nrows <= 0 || n <= 0) return;
     const int ntj = (n + 63) / 64;
     const int64_t nrt = (nrows + 255) / 256;
-    k_lincomb_mfma<<<(unsigned)(nrt * ntj), kWG, 0, s>>>(nrows, m, n, X, ldx, Ct, ldct,
-                                                        beta_mode, beta_scalar, beta, Y, ldy,
-                                                        ntj);
+    const unsigned g = (unsigned)(nrt * ntj);
+    if (beta_mode == 0)
+        k_lincomb_mfma<0><<<g, kWG, 0, s>>>(nrows, m, n, X, ldx, Ct, ldct, beta_scalar, beta, Y,
+                                            ldy, ntj);
+    else if (beta_mode == 1)
+        k_lincomb_mfma<1><<<g, kWG, 0, s>>>(nrows, m, n, X, ldx, Ct, ldct, beta_scalar, beta, Y,
+                                            ldy, ntj);
+    else
+        k_lincomb_mfma<2><<<g, kWG, 0, s>>>(nrows, m, n, X, ldx, Ct, ldct, beta_scalar, beta, Y,
+                                            ldy, ntj);
 }
 
 }  // namespace gcge
