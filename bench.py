@@ -161,6 +161,78 @@ def rocprof_summary(d: str):
             print(f"{cnt:10d} {c:>12} {tot / cnt:16.1f}  {k}")
 
 
+# kernel class of the library profiler -> device kernel name prefix
+KERNEL_OF_CLASS = {"lincomb": "k_lincomb_mfma", "gram": "k_gram_mfma", "spmm64": "k_spmm_sell",
+                   "spmm32": "k_spmm_pw"}
+
+
+def _counter_means(path: str):
+    """{kernel name: (dispatches, mean)} from a --rocprof-summary output."""
+    out = {}
+    for ln in open(path):
+        f = ln.split(None, 3)
+        if len(f) == 4 and f[0].isdigit() and f[1] in ("FETCH_SIZE", "WRITE_SIZE"):
+            out[f[3].strip()] = (int(f[0]), float(f[2]))
+    return out
+
+
+def _short(kernel: str) -> str:
+    """'void gcge::(anonymous namespace)::k_x<8>(long, ...)' -> 'k_x<8>'."""
+    k = kernel.replace("(anonymous namespace)::", "")
+    if k.startswith("void "):
+        k = k[5:]
+    return k.split("(")[0].split("::")[-1]
+
+
+def _find(means: dict, prefix: str):
+    hits = [(k, v) for k, v in means.items()
+            if _short(k).split("<")[0] == prefix.split("<")[0]
+            and (("<" not in prefix) or _short(k) == prefix)]
+    if not hits:
+        return None
+    # several template instances: dispatch-weighted mean
+    n = sum(v[0] for _, v in hits)
+    return n, sum(v[0] * v[1] for _, v in hits) / n
+
+
+def pmc_traffic(fetch: str, write: str, cfetch: str, cwrite: str, bench_json: str):
+    """HBM bytes per launch from FETCH_SIZE/WRITE_SIZE summaries (KiB per
+    dispatch), each scaled by what the counter reads for the copy probe of
+    known size, next to the algorithmic bytes of the bench line."""
+    copy_bytes = 120 * 2097152 * 8
+    mf, mw = _counter_means(fetch), _counter_means(write)
+    cf = _find(_counter_means(cfetch), "k_copy_cols")[1]
+    cw = _find(_counter_means(cwrite), "k_copy_cols")[1]
+    ff, fw = copy_bytes / (cf * 1024), copy_bytes / (cw * 1024)
+    line = json.loads(open(bench_json).read().strip().splitlines()[-1])
+    kern = line["paths"]["mixed"]["kernels"]
+    print("# calibration: the copy probe (k_copy_cols) reads and writes "
+          f"{copy_bytes:,} B per launch")
+    print(f"#   FETCH_SIZE reads {cf:.1f} KiB -> factor {ff:.4f}")
+    print(f"#   WRITE_SIZE reads {cw:.1f} KiB -> factor {fw:.4f}")
+    print("# per kernel class of the bench line (mean over dispatches): FETCH_SIZE KiB, "
+          "WRITE_SIZE KiB, calibrated bytes, algorithmic bytes")
+    print(f"{'class':8s} {'kernel':16s} {'dispatches':>10} {'FETCH KiB':>12} {'WRITE KiB':>12} "
+          f"{'measured B':>12} {'algorithmic B':>13} {'ratio':>6}")
+    result = {}
+    for cls, kname in KERNEL_OF_CLASS.items():
+        if cls not in kern:
+            continue
+        a, b = _find(mf, kname), _find(mw, kname)
+        if a is None or b is None:
+            continue
+        meas = a[1] * 1024 * ff + b[1] * 1024 * fw
+        alg = kern[cls]["bytes"] / kern[cls]["launches"]
+        result[cls] = meas
+        print(f"{cls:8s} {kname:16s} {a[0]:10d} {a[1]:12.1f} {b[1]:12.1f} {meas:12.4e} "
+              f"{alg:13.4e} {meas / alg:6.3f}")
+    print("# top kernels by calibrated bytes per dispatch")
+    for k in sorted(set(mf) & set(mw), key=lambda k: -(mf[k][1] * ff + mw[k][1] * fw))[:16]:
+        print(f"{mf[k][0]:10d} {mf[k][1] * 1024 * ff + mw[k][1] * 1024 * fw:12.4e}  {_short(k)}")
+    for cls, meas in result.items():
+        print(f"traffic_bytes_per_launch {cls} {meas:.6e}")
+
+
 def measured_traffic(kernel: str):
     """HBM bytes per launch of `kernel` from the committed counter summary:
     a line 'traffic_bytes_per_launch <kernel> <bytes>'."""
@@ -216,9 +288,14 @@ def main():
     ap.add_argument("--cpu-iters", type=int, default=1)
     ap.add_argument("--copy-probe", action="store_true")
     ap.add_argument("--rocprof-summary", metavar="DIR")
+    ap.add_argument("--pmc-traffic", nargs=5,
+                    metavar=("FETCH", "WRITE", "COPY_FETCH", "COPY_WRITE", "BENCH_JSON"))
     args = ap.parse_args()
     if args.rocprof_summary:
         rocprof_summary(args.rocprof_summary)
+        return
+    if args.pmc_traffic:
+        pmc_traffic(*args.pmc_traffic)
         return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1",
