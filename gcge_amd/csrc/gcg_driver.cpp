@@ -135,9 +135,9 @@ void orth_self(void **x, int start_x, int *end_x, void *B, const OrthParams &p, 
 // Removes from x[:, start_x:end_x] its components along x[:, :start_x] in
 // the B inner product: block classical Gram-Schmidt passes until the
 // largest coefficient is below reorth_tol, at most 1 + max_reorth
-// (src/ops_orth.c:231-269).
+// (src/ops_orth.c:231-269), or at most max_passes when that is positive.
 void remove_previous(void **x, int start_x, int end_x, void *B, const OrthParams &p,
-                     void **mv_ws, OPS *ops)
+                     void **mv_ws, OPS *ops, int max_passes = 0)
 {
     if (start_x <= 0 || end_x <= start_x) return;
     int s[2], e[2];
@@ -146,7 +146,9 @@ void remove_previous(void **x, int start_x, int end_x, void *B, const OrthParams
     s[1] = start_x; e[1] = end_x;
     const int64_t len = (int64_t)(e[1] - s[1]) * (e[0] - s[0]);
     std::vector<double> coef((size_t)len);
-    for (int it = 0; it < 1 + p.max_reorth; ++it) {
+    int passes = 1 + p.max_reorth;
+    if (max_passes > 0) passes = std::min(passes, max_passes);
+    for (int it = 0; it < passes; ++it) {
         ops->MultiVecQtAP('S', 'N', x, B, x, 0, s, e, coef.data(), e[0] - s[0], mv_ws, ops);
         double amax = 0.0;
         for (int64_t i = 0; i < len; ++i) {
@@ -199,7 +201,11 @@ void cholqr(void **x, int start_x, int *end_x, void *B, const OrthParams &p, voi
             OPS *ops, int64_t *fallbacks)
 {
     if (*end_x <= start_x) return;
-    remove_previous(x, start_x, *end_x, B, p, mv_ws, ops);
+    // two block passes ("twice is enough" for classical Gram-Schmidt: the
+    // second pass leaves components of order eps * ||x||, Giraud, Langou
+    // and Rozloznik 2005); the reference's absolute test on the largest
+    // coefficient (50 eps) would run a third pass on unit-size columns
+    remove_previous(x, start_x, *end_x, B, p, mv_ws, ops, 2);
     const int n = *end_x - start_x;
     std::vector<double> G((size_t)n * n), R, Rinv((size_t)n * n), dinv((size_t)n);
     for (int pass = 0; pass < 2; ++pass) {
