@@ -29,19 +29,58 @@ inline mv *as_mv(void **p) { return reinterpret_cast<mv *>(p); }
 
 double *col(mv *x, int c) { return x->data + (size_t)x->ldd * (size_t)c; }
 
+// a . b over k entries with four independent partial sums, so the sum is
+// not one dependency chain (a single accumulator runs at the latency of
+// one FMA per element: the compiler may not reassociate it)
+double dot(const double *a, const double *b, int k)
+{
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    int r = 0;
+    for (; r + 4 <= k; r += 4) {
+        s0 += a[r] * b[r];
+        s1 += a[r + 1] * b[r + 1];
+        s2 += a[r + 2] * b[r + 2];
+        s3 += a[r + 3] * b[r + 3];
+    }
+    for (; r < k; ++r) s0 += a[r] * b[r];
+    return (s0 + s1) + (s2 + s3);
+}
+
 // C[0:mc, 0:nc] (ldC) = alpha * Q^T P + beta * C, Q: k x mc (ldQ), P: k x nc.
+// Four columns of Q share each pass over a column of P, two partial sums
+// each.
 void gemm_tn(int mc, int nc, int k, double alpha, const double *Q, int ldQ,
              const double *P, int ldP, double beta, double *C, int ldC)
 {
+    auto put = [&](int i, int j, double v) {
+        double *c = C + (size_t)ldC * j + i;
+        *c = (beta == 0.0) ? alpha * v : alpha * v + beta * *c;
+    };
     for (int j = 0; j < nc; ++j) {
         const double *p = P + (size_t)ldP * j;
-        for (int i = 0; i < mc; ++i) {
-            const double *q = Q + (size_t)ldQ * i;
-            double s = 0.0;
-            for (int r = 0; r < k; ++r) s += q[r] * p[r];
-            double *c = C + (size_t)ldC * j + i;
-            *c = (beta == 0.0) ? alpha * s : alpha * s + beta * *c;
+        int i = 0;
+        for (; i + 4 <= mc; i += 4) {
+            const double *q0 = Q + (size_t)ldQ * i, *q1 = q0 + ldQ, *q2 = q1 + ldQ,
+                         *q3 = q2 + ldQ;
+            double a0 = 0, a1 = 0, a2 = 0, a3 = 0, b0 = 0, b1 = 0, b2 = 0, b3 = 0;
+            int r = 0;
+            for (; r + 2 <= k; r += 2) {
+                const double p0 = p[r], p1 = p[r + 1];
+                a0 += q0[r] * p0; b0 += q0[r + 1] * p1;
+                a1 += q1[r] * p0; b1 += q1[r + 1] * p1;
+                a2 += q2[r] * p0; b2 += q2[r + 1] * p1;
+                a3 += q3[r] * p0; b3 += q3[r + 1] * p1;
+            }
+            if (r < k) {
+                a0 += q0[r] * p[r]; a1 += q1[r] * p[r];
+                a2 += q2[r] * p[r]; a3 += q3[r] * p[r];
+            }
+            put(i, j, a0 + b0);
+            put(i + 1, j, a1 + b1);
+            put(i + 2, j, a2 + b2);
+            put(i + 3, j, a3 + b3);
         }
+        for (; i < mc; ++i) put(i, j, dot(Q + (size_t)ldQ * i, p, k));
     }
 }
 
@@ -88,8 +127,7 @@ void gcge_host_DenseMatQtAP(char ntluA, char nsdC, int nrowsA, int ncolsA,
         if (nsdC == 'D') {
             for (int c = 0; c < ncolsC; ++c) {
                 const double *q = matQ + (size_t)ldQ * c, *p = matP + (size_t)ldP * c;
-                double s = 0.0;
-                for (int r = 0; r < k; ++r) s += q[r] * p[r];
+                const double s = dot(q, p, k);
                 double *d = matC + (size_t)ldC * c;
                 if (beta == 1.0) {
                     if (alpha != 0.0) *d += alpha * s;
@@ -103,9 +141,7 @@ void gcge_host_DenseMatQtAP(char ntluA, char nsdC, int nrowsA, int ncolsA,
             for (int c = 0; c < ncolsC; ++c) {
                 const double *p = matP + (size_t)ldP * c;
                 for (int r = c; r < ncolsC; ++r) {
-                    const double *q = matQ + (size_t)ldQ * r;
-                    double s = 0.0;
-                    for (int t = 0; t < k; ++t) s += q[t] * p[t];
+                    const double s = dot(matQ + (size_t)ldQ * r, p, k);
                     double *d = matC + (size_t)ldC * c + r;
                     *d = (beta == 0.0) ? alpha * s : alpha * s + beta * *d;
                 }
@@ -138,19 +174,11 @@ void gcge_host_DenseMatQtAP(char ntluA, char nsdC, int nrowsA, int ncolsA,
                 if (ntluA == 'L') {
                     // column c holds A(c..n-1, c); A(c, r) = A(r, c) for r > c
                     w[c] += a[c] * pc;
-                    double s = 0.0;
-                    for (int r = c + 1; r < n; ++r) {
-                        w[r] += a[r] * pc;
-                        s += a[r] * p[r];
-                    }
-                    w[c] += s;
+                    for (int r = c + 1; r < n; ++r) w[r] += a[r] * pc;
+                    w[c] += dot(a + c + 1, p + c + 1, n - c - 1);
                 } else {
-                    double s = 0.0;
-                    for (int r = 0; r < c; ++r) {
-                        w[r] += a[r] * pc;
-                        s += a[r] * p[r];
-                    }
-                    w[c] += a[c] * pc + s;
+                    for (int r = 0; r < c; ++r) w[r] += a[r] * pc;
+                    w[c] += a[c] * pc + dot(a, p, c);
                 }
             }
         }

@@ -259,8 +259,9 @@ ThinPlan thin_plan(int64_t nrows, int m, int n)
     p.mt = 64 / p.nb;
     p.mtiles = (m + p.mt - 1) / p.mt;
     // a few long-lived workgroups per CU: the register accumulators are
-    // reduced once per workgroup, so each should stream many rows
-    int64_t want = (512 + p.mtiles - 1) / p.mtiles;
+    // reduced once per workgroup, so each should stream many rows (at most
+    // 512 in total, so no straggler round)
+    int64_t want = 512 / p.mtiles;
     int64_t cap = (nrows + 8191) / 8192;
     int64_t nc = want < cap ? want : cap;
     if (nc < 1) nc = 1;

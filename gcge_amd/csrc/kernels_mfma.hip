@@ -35,18 +35,20 @@ __device__ __forceinline__ d4 mfma(double a, double b, d4 c)
 // ---------------------------------------------------------------------
 // Gram: G (m x n) = X^T Y.  A workgroup owns a 128 x 128 tile of G over one
 // chunk of rows; its four waves own 64 x 64 quadrants (4 x 4 MFMA tiles).
-// Rows are staged 32 at a time through LDS, stored column-major (row index
-// fastest) so the cooperative global loads and the LDS writes are
-// contiguous and each 16-lane MFMA operand read covers the 32 banks once.
-// The next stage's global loads are issued into registers before the
-// current stage's MFMAs, so HBM latency overlaps the matrix-core work.
+// Rows are staged 16 at a time through LDS, stored column-major (row index
+// fastest, padded to 17) so each 16-lane MFMA operand read covers the
+// banks once.  Each thread fetches 4 consecutive rows (one 32-byte load)
+// of 4 columns per operand, and the next stage's loads are issued into
+// registers before the current stage's MFMAs, so HBM latency overlaps the
+// matrix-core work; at 98 VGPRs + 128 AGPRs two waves share each SIMD.
 // Partial tiles per chunk are summed by k_gram_mfma_reduce in a fixed
 // order (deterministic, no atomics).
 // ---------------------------------------------------------------------
 constexpr int kGT = 128;
-constexpr int kGK = 32;
+constexpr int kGK = 16;                 // rows per stage
 constexpr int kGKP = kGK + 1;
-constexpr int kGL = (kGK * kGT) / kWG;  // elements per thread per operand per stage
+constexpr int kGRG = kGK / 4;           // 4-row groups per stage
+constexpr int kGCT = kGT * kGRG / kWG;  // columns per thread per operand
 
 __global__ __launch_bounds__(kWG) void k_gram_mfma(
     int64_t nrows, int m, int n, const double *__restrict__ X, int64_t ldx,
@@ -72,18 +74,48 @@ __global__ __launch_bounds__(kWG) void k_gram_mfma(
     const int64_t rbeg = (int64_t)chunk * rows_per_chunk;
     int64_t rend = rbeg + rows_per_chunk;
     if (rend > nrows) rend = nrows;
-    // this thread's slots in a stage: row rr = e % 32, column cc = e / 32
-    const int rr = threadIdx.x % kGK;
-    const int cc0 = threadIdx.x / kGK;  // + 8 * q
-    double px[kGL], py[kGL];
-    auto fetch = [&](int64_t r0) {
-        const int64_t r = r0 + rr;
-        const bool rok = r < rend;
+    // this thread's slots in a stage: rows 4*rg .. 4*rg+3 of columns
+    // kGCT*cg .. kGCT*cg+kGCT-1 (one 32-byte load per column; kGRG lanes
+    // cover the stage's rows of a column, so a wave instruction reads whole
+    // row segments of 64/kGRG columns, and only kGCT column addresses per
+    // operand stay live)
+    const int rg = threadIdx.x % kGRG;
+    const int cg = threadIdx.x / kGRG;
+    const double *xc[kGCT], *yc[kGCT];
+    bool xok[kGCT], yok[kGCT];
 #pragma unroll
-        for (int q = 0; q < kGL; ++q) {
-            const int cc = cc0 + q * (kWG / kGK);
-            px[q] = (rok && i0 + cc < m) ? X[r + (int64_t)(i0 + cc) * ldx] : 0.0;
-            py[q] = (rok && j0 + cc < n) ? Y[r + (int64_t)(j0 + cc) * ldy] : 0.0;
+    for (int c = 0; c < kGCT; ++c) {
+        const int cc = cg * kGCT + c;
+        xok[c] = i0 + cc < m;
+        yok[c] = j0 + cc < n;
+        xc[c] = X + (int64_t)(xok[c] ? i0 + cc : 0) * ldx + 4 * rg;
+        yc[c] = Y + (int64_t)(yok[c] ? j0 + cc : 0) * ldy + 4 * rg;
+    }
+    double px[kGCT][4], py[kGCT][4];
+    auto fetch = [&](int64_t r0) {
+        const int64_t r = r0 + 4 * rg;
+        if (r + 4 <= rend) {
+#pragma unroll
+            for (int c = 0; c < kGCT; ++c) {
+                const d4 vx = xok[c] ? *reinterpret_cast<const d4 *>(xc[c] + r0)
+                                     : d4{0.0, 0.0, 0.0, 0.0};
+                const d4 vy = yok[c] ? *reinterpret_cast<const d4 *>(yc[c] + r0)
+                                     : d4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+                for (int t = 0; t < 4; ++t) {
+                    px[c][t] = vx[t];
+                    py[c][t] = vy[t];
+                }
+            }
+        } else {
+#pragma unroll
+            for (int c = 0; c < kGCT; ++c)
+#pragma unroll
+                for (int t = 0; t < 4; ++t) {
+                    const bool rok = r + t < rend;
+                    px[c][t] = (rok && xok[c]) ? xc[c][r0 + t] : 0.0;
+                    py[c][t] = (rok && yok[c]) ? yc[c][r0 + t] : 0.0;
+                }
         }
     };
     d4 acc[4][4];
@@ -94,14 +126,15 @@ __global__ __launch_bounds__(kWG) void k_gram_mfma(
     if (rbeg < rend) fetch(rbeg);
     for (int64_t r0 = rbeg; r0 < rend; r0 += kGK) {
 #pragma unroll
-        for (int q = 0; q < kGL; ++q) {
-            const int cc = cc0 + q * (kWG / kGK);
-            sx[cc * kGKP + rr] = px[q];
-            sy[cc * kGKP + rr] = py[q];
-        }
+        for (int c = 0; c < kGCT; ++c)
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                sx[(cg * kGCT + c) * kGKP + 4 * rg + t] = px[c][t];
+                sy[(cg * kGCT + c) * kGKP + 4 * rg + t] = py[c][t];
+            }
         __syncthreads();
         if (r0 + kGK < rend) fetch(r0 + kGK);
-#pragma unroll
+#pragma unroll 2
         for (int kk = 0; kk < kGK / 4; ++kk) {
             const int row = kk * 4 + lk;
             double av[4], bv[4];
@@ -158,7 +191,9 @@ GramPlanM plan(int64_t nrows, int m, int n)
     p.tm = (m + kGT - 1) / kGT;
     p.tn = (n + kGT - 1) / kGT;
     const int64_t tiles = (int64_t)p.tm * p.tn;
-    int64_t want = (512 + tiles - 1) / tiles;   // two workgroups per CU
+    // at most two workgroups per CU in total (256 CUs x 2 resident): one
+    // more than that runs alone in a second round and doubles the time
+    int64_t want = 512 / tiles;
     int64_t cap = (nrows + 4095) / 4096;        // at least 4096 rows per chunk
     int64_t nc = want < cap ? want : cap;
     if (nc < 1) nc = 1;
