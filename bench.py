@@ -80,7 +80,7 @@ def run_path(prob, n, dim, mode, steps, warmup, mats, want_check=True):
     lib.gcge_hip_profile_enable(0)
     prof = {}
     import ctypes as C
-    for kind, name in enumerate(("spmm64", "spmm32", "gram", "lincomb", "blas1")):
+    for kind, name in enumerate(("spmm64", "spmm32", "gram", "lincomb", "blas1", "pcgvec")):
         la, ms, by, fl = C.c_int64(), C.c_double(), C.c_double(), C.c_double()
         lib.gcge_hip_profile_read(kind, C.byref(la), C.byref(ms), C.byref(by), C.byref(fl))
         if la.value:
@@ -162,8 +162,9 @@ def rocprof_summary(d: str):
 
 
 # kernel class of the library profiler -> device kernel name prefix
-KERNEL_OF_CLASS = {"lincomb": "k_lincomb_mfma", "gram": "k_gram_mfma", "spmm64": "k_spmm_sell",
-                   "spmm32": "k_spmm_pw"}
+# (one profiler launch of these classes is one dispatch of one of the kernels)
+KERNEL_OF_CLASS = {"lincomb": ["k_lincomb_mfma"], "spmm64": ["k_spmm_sell"],
+                   "spmm32": ["k_spmm_pw"], "pcgvec": ["k_update_p", "k_update_xr"]}
 
 
 def _counter_means(path: str):
@@ -184,10 +185,11 @@ def _short(kernel: str) -> str:
     return k.split("(")[0].split("::")[-1]
 
 
-def _find(means: dict, prefix: str):
-    hits = [(k, v) for k, v in means.items()
-            if _short(k).split("<")[0] == prefix.split("<")[0]
-            and (("<" not in prefix) or _short(k) == prefix)]
+def _find(means: dict, prefix):
+    prefixes = [prefix] if isinstance(prefix, str) else prefix
+    hits = [(k, v) for k, v in means.items() for pf in prefixes
+            if _short(k).split("<")[0] == pf.split("<")[0]
+            and (("<" not in pf) or _short(k) == pf)]
     if not hits:
         return None
     # several template instances: dispatch-weighted mean
@@ -212,7 +214,7 @@ def pmc_traffic(fetch: str, write: str, cfetch: str, cwrite: str, bench_json: st
     print(f"#   WRITE_SIZE reads {cw:.1f} KiB -> factor {fw:.4f}")
     print("# per kernel class of the bench line (mean over dispatches): FETCH_SIZE KiB, "
           "WRITE_SIZE KiB, calibrated bytes, algorithmic bytes")
-    print(f"{'class':8s} {'kernel':16s} {'dispatches':>10} {'FETCH KiB':>12} {'WRITE KiB':>12} "
+    print(f"{'class':8s} {'kernels':24s} {'dispatches':>10} {'FETCH KiB':>12} {'WRITE KiB':>12} "
           f"{'measured B':>12} {'algorithmic B':>13} {'ratio':>6}")
     result = {}
     for cls, kname in KERNEL_OF_CLASS.items():
@@ -224,7 +226,7 @@ def pmc_traffic(fetch: str, write: str, cfetch: str, cwrite: str, bench_json: st
         meas = a[1] * 1024 * ff + b[1] * 1024 * fw
         alg = kern[cls]["bytes"] / kern[cls]["launches"]
         result[cls] = meas
-        print(f"{cls:8s} {kname:16s} {a[0]:10d} {a[1]:12.1f} {b[1]:12.1f} {meas:12.4e} "
+        print(f"{cls:8s} {'+'.join(kname):24s} {a[0]:10d} {a[1]:12.1f} {b[1]:12.1f} {meas:12.4e} "
               f"{alg:13.4e} {meas / alg:6.3f}")
     print("# top kernels by calibrated bytes per dispatch")
     for k in sorted(set(mf) & set(mw), key=lambda k: -(mf[k][1] * ff + mw[k][1] * fw))[:16]:

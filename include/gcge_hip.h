@@ -108,7 +108,8 @@ int gcge_hip_device(void);
  * and every result written once).
  *   kind 0: FP64 SpMM (MatDotMultiVec), 1: FP32 shifted SpMM of the
  *   device BlockPCG, 2: Gram (inner products, both kernels),
- *   3: LinearComb, 4: BLAS-1. */
+ *   3: LinearComb, 4: BLAS-1, 5: vector updates of the device BlockPCG
+ *   (p = r + beta p; x += alpha p, r -= alpha w with r.r). */
 void gcge_hip_profile_enable(int on);
 void gcge_hip_profile_reset(void);
 void gcge_hip_profile_read(int kind, int64_t *launches, double *total_ms, double *bytes,
