@@ -94,10 +94,28 @@ class DenseMV(C.Structure):
     _fields_ = [("data", c_dbl_p), ("nrows", C.c_int), ("ncols", C.c_int), ("ldd", C.c_int)]
 
 
+class Request(C.Structure):
+    """gcge_request (include/gcge_hip.h)."""
+    _fields_ = [("A", C.POINTER(CCSMat)), ("B", C.POINTER(CCSMat)), ("nev", C.c_int),
+                ("block_size", C.c_int), ("nev_init", C.c_int), ("nev_max", C.c_int),
+                ("max_iter", C.c_int), ("mode", C.c_int), ("shift", C.c_int),
+                ("tol_abs", C.c_double), ("tol_rel", C.c_double), ("gap_min", C.c_double),
+                ("seed", C.c_uint64)]
+
+
+class Result(C.Structure):
+    """gcge_result (include/gcge_hip.h)."""
+    _fields_ = [("status", C.c_int), ("nev_conv", C.c_int), ("num_iter", C.c_int),
+                ("concurrent", C.c_int), ("queue_seconds", C.c_double),
+                ("setup_seconds", C.c_double), ("solve_seconds", C.c_double),
+                ("total_seconds", C.c_double), ("eval", c_dbl_p), ("evec", c_dbl_p)]
+
+
 # Exported functions of libgcge_hip.so and their signatures.
 EXPORTS = {
     "OPS_HIP_Set": (None, [OPS_p]),
     "OPS_HOSTDENSE_Set": (None, [OPS_p]),
+    "OPS_HIP_Clear": (None, [OPS_p]),
     "gcge_host_DenseMatQtAP": (None, [C.c_char, C.c_char, C.c_int, C.c_int, C.c_int, C.c_int,
                                       C.c_double, c_dbl_p, C.c_int, c_dbl_p, C.c_int, c_dbl_p,
                                       C.c_int, C.c_double, c_dbl_p, C.c_int, c_dbl_p]),
@@ -118,6 +136,7 @@ EXPORTS = {
                                 C.POINTER(C.c_int64)]),
     "gcge_hip_set_random_mode": (None, [C.c_int, C.c_uint64]),
     "gcge_hip_synchronize": (None, []),
+    "gcge_hip_thread_release": (None, []),
     "gcge_hip_last_error": (C.c_int, [C.c_char_p, C.c_int]),
     "gcge_hip_device": (C.c_int, []),
     "gcge_hip_profile_enable": (None, [C.c_int]),
@@ -142,6 +161,14 @@ EXPORTS = {
     "HIPGCG_GetOrthStats": (None, [OPS_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
     "HIPGCG_MultiLinearSolverSetup_BlockPCG32": (None, [C.c_int, C.c_double, C.c_double,
                                                         C.c_void_p, C.c_double, OPS_p]),
+    "HIPGCG_MultiLinearSolverFree_BlockPCG32": (None, [OPS_p]),
+    "gcge_request_footprint": (C.c_int64, [C.POINTER(Request)]),
+    "gcge_service_create": (C.c_void_p, [C.c_int, C.c_int64]),
+    "gcge_service_submit": (C.c_int64, [C.c_void_p, C.POINTER(Request), C.POINTER(Result)]),
+    "gcge_service_wait": (C.c_int, [C.c_void_p, C.c_int64]),
+    "gcge_service_stats": (None, [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int),
+                                  C.POINTER(C.c_int64)]),
+    "gcge_service_destroy": (None, [C.c_void_p]),
 }
 
 GCGE_HIP_MAT_FP32 = 1
@@ -151,6 +178,8 @@ GCGE_EIG_HOST = 0
 GCGE_EIG_DEVICE = 1
 GCGE_ORTH_MGS = 0
 GCGE_ORTH_CHOLQR = 1
+GCGE_MODE_FP64 = 0
+GCGE_MODE_MIXED = 1
 
 _lib = None
 

@@ -51,15 +51,35 @@ void hip_check(hipError_t e, const char *expr, const char *file, int line)
 }
 
 // ----------------------------------------------------------------- context
+namespace {
+thread_local Ctx *t_ctx = nullptr;
+}
+
 Ctx &ctx()
 {
-    static Ctx *c = [] {
+    if (t_ctx == nullptr) {
         Ctx *x = new Ctx();
         HIPCHECK(hipGetDevice(&x->device));
         HIPCHECK(hipStreamCreateWithFlags(&x->stream, hipStreamNonBlocking));
-        return x;
-    }();
-    return *c;
+        t_ctx = x;
+    }
+    return *t_ctx;
+}
+
+void release_ctx()
+{
+    Ctx *c = t_ctx;
+    if (c == nullptr) return;
+    c->sync();
+    c->prof.collect();
+    if (c->dev_eig) release_dev_eig(c->dev_eig);
+    if (c->d_scratch) HIPCHECK(hipFree(c->d_scratch));
+    if (c->d_small) HIPCHECK(hipFree(c->d_small));
+    if (c->h_pinned) HIPCHECK(hipHostFree(c->h_pinned));
+    for (hipEvent_t e : c->prof.pool) HIPCHECK(hipEventDestroy(e));
+    HIPCHECK(hipStreamDestroy(c->stream));
+    delete c;
+    t_ctx = nullptr;
 }
 
 double *Ctx::scratch(int64_t ndouble)
@@ -192,6 +212,8 @@ MV *mv_create(int64_t nrows, int64_t ncols)
     const size_t bytes = (size_t)(nrows * (ncols > 0 ? ncols : 1)) * sizeof(double);
     HIPCHECK(hipMalloc(reinterpret_cast<void **>(&m->d), bytes));
     HIPCHECK(hipMemsetAsync(m->d, 0, bytes, c.stream));
+    // another thread's stream may use the multivector next
+    c.sync();
     return m;
 }
 
@@ -554,6 +576,14 @@ void h_MatDotVec(void *m, void *x, void *y, OPS *ops)
 // ------------------------------------------------------------------ C ABI
 extern "C" {
 
+void OPS_HIP_Clear(OPS *ops)
+{
+    if (ops->lapack_ops) {
+        free(ops->lapack_ops);
+        ops->lapack_ops = nullptr;
+    }
+}
+
 void OPS_HIP_Set(OPS *ops)
 {
     if (ops->lapack_ops != nullptr)
@@ -737,6 +767,8 @@ void gcge_hip_set_random_mode(int mode, uint64_t seed)
 }
 
 void gcge_hip_synchronize(void) { ctx().sync(); }
+
+void gcge_hip_thread_release(void) { release_ctx(); }
 
 int gcge_hip_last_error(char *buf, int len)
 {

@@ -41,8 +41,10 @@ struct Profiler {
     void reset();
 };
 
-// One device, one stream per process; every slot enqueues on `stream` and
-// only slots that return host data synchronize.
+// One context per host thread: a stream and the thread's staging buffers.
+// Every slot enqueues on the calling thread's stream and only slots that
+// return host data synchronize, so tables driven from different threads
+// run concurrently on the device.
 struct Ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -57,12 +59,17 @@ struct Ctx {
     uint64_t rand_counter = 0;
     Stats stats;
     Profiler prof;
+    // the device dense eigensolver's handle and buffers (dense_eig.cpp)
+    void *dev_eig = nullptr;
     double *scratch(int64_t ndouble);
     double *small(int64_t ndouble);
     double *pinned(int64_t ndouble);
     void sync();
 };
 Ctx &ctx();
+// Frees the calling thread's context (a later call creates a new one).
+void release_ctx();
+void release_dev_eig(void *p);
 
 // Device multivector: column-major nrows x ncols, leading dimension ldd.
 struct MV {

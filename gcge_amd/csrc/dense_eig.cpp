@@ -82,6 +82,8 @@ int eigh_host(int n, double *A, int lda, double *w, double *Z, int ldz)
 }
 
 namespace {
+// per thread (held by the thread's context): a rocBLAS handle on the
+// thread's stream and the device copies of the projected matrix
 struct DevEig {
     rocblas_handle h = nullptr;
     double *dA = nullptr, *dW = nullptr, *dE = nullptr;
@@ -90,15 +92,30 @@ struct DevEig {
 };
 DevEig &deveig()
 {
-    static DevEig d;
-    if (d.h == nullptr) {
-        if (rocblas_create_handle(&d.h) != rocblas_status_success)
+    Ctx &c = ctx();
+    if (c.dev_eig == nullptr) {
+        DevEig *d = new DevEig();
+        if (rocblas_create_handle(&d->h) != rocblas_status_success)
             fail("rocblas_create_handle failed", __FILE__, __LINE__);
-        rocblas_set_stream(d.h, ctx().stream);
+        rocblas_set_stream(d->h, c.stream);
+        c.dev_eig = d;
     }
-    return d;
+    return *static_cast<DevEig *>(c.dev_eig);
 }
 }  // namespace
+
+void release_dev_eig(void *p)
+{
+    DevEig *d = static_cast<DevEig *>(p);
+    if (d->dA) {
+        HIPCHECK(hipFree(d->dA));
+        HIPCHECK(hipFree(d->dW));
+        HIPCHECK(hipFree(d->dE));
+        HIPCHECK(hipFree(d->dinfo));
+    }
+    rocblas_destroy_handle(d->h);
+    delete d;
+}
 
 int eigh_device(int n, double *A, int lda, double *w, double *Z, int ldz)
 {

@@ -942,6 +942,7 @@ void HIPGCG_Destroy(void ***mv_ws, double *dbl_ws, int *int_ws, int nevMax, int 
 
 void HIPGCG_Free(OPS *ops)
 {
+    HIPGCG_MultiLinearSolverFree_BlockPCG32(ops);
     if (ops->EigenSolver == gcg && ops->eigen_solver_workspace) {
         delete static_cast<Solver *>(ops->eigen_solver_workspace);
         ops->eigen_solver_workspace = nullptr;

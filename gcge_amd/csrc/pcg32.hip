@@ -576,6 +576,18 @@ void pcg32_solve(void *matA, void **mv_b, void **mv_x, int *start, int *end, OPS
 }  // namespace
 }  // namespace gcge
 
+extern "C" void HIPGCG_MultiLinearSolverFree_BlockPCG32(OPS *ops)
+{
+    using namespace gcge;
+    if (ops->MultiLinearSolver != pcg32_solve || ops->multi_linear_solver_workspace == nullptr)
+        return;
+    PCG32 *S = static_cast<PCG32 *>(ops->multi_linear_solver_workspace);
+    S->release();
+    delete S;
+    ops->multi_linear_solver_workspace = nullptr;
+    ops->MultiLinearSolver = nullptr;
+}
+
 extern "C" void HIPGCG_MultiLinearSolverSetup_BlockPCG32(int max_iter, double rate, double tol,
                                                          void *B, double sigma, OPS *ops)
 {

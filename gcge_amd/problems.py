@@ -140,3 +140,81 @@ def config2(block_size: int = 120) -> CCSProblem:
         return stencil_problem(128, 3, 100, 120, 220, 220, name="cfg2_3d_128")
     bs, ni, nm = slepc_block_policy(100)
     return stencil_problem(128, 3, 100, bs, ni, nm, name="cfg2_3d_128_slepc")
+
+
+SEED_BASE = 20261202
+
+
+def varcoef_pair_2d(n: int, sigma: float, rng: np.random.Generator):
+    """Variable-coefficient diffusion on an n x n interior grid, Dirichlet
+    boundary, 5-point stencil, index x + n*y.
+
+    Node coefficients kappa = exp(sigma * z), z ~ N(0, 1); the weight of
+    the edge between neighbours i, j is the harmonic mean
+    2 kappa_i kappa_j / (kappa_i + kappa_j); A_ij = -w_ij and A_ii is the
+    sum of the weights at i, where a missing (boundary) neighbour adds
+    kappa_i.  B = (8 I + Adj) / 12 as in the constant-coefficient pair, to
+    which kappa = 1 reduces.  A is symmetric and strictly diagonally
+    dominant, hence SPD."""
+    N = n * n
+    kappa = np.exp(sigma * rng.standard_normal(N))
+    idx = np.arange(N).reshape(n, n)          # [y, x]
+    rows, cols, vals = [], [], []
+    diag = np.zeros(N)
+    for a, b in ((idx[:, :-1].ravel(), idx[:, 1:].ravel()),    # x-neighbours
+                 (idx[:-1, :].ravel(), idx[1:, :].ravel())):   # y-neighbours
+        w = 2.0 * kappa[a] * kappa[b] / (kappa[a] + kappa[b])
+        rows += [a, b]
+        cols += [b, a]
+        vals += [-w, -w]
+        np.add.at(diag, a, w)
+        np.add.at(diag, b, w)
+    nb = np.zeros((n, n))
+    nb[:, 0] += 1
+    nb[:, -1] += 1
+    nb[0, :] += 1
+    nb[-1, :] += 1
+    diag += nb.ravel() * kappa
+    rows.append(np.arange(N))
+    cols.append(np.arange(N))
+    vals.append(diag)
+    A = sp.coo_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))),
+                      shape=(N, N))
+    B = (8.0 * sp.identity(N) + grid_adjacency(n, 2)) / 12.0
+    return _canonical_csc(A), _canonical_csc(B)
+
+
+def config4_problem(i: int, seed: int = SEED_BASE) -> CCSProblem:
+    """Problem i of the batched set: N ~ U[2048, 65536], grid side
+    round(sqrt(N)), variable coefficients lognormal(0, 0.5),
+    nev ~ U{8..64}, SLEPc block policy, tol 1e-8."""
+    rng = np.random.default_rng([seed, i])
+    N = int(rng.integers(2048, 65536 + 1))
+    n = int(round(math.sqrt(N)))
+    nev = int(rng.integers(8, 64 + 1))
+    A, B = varcoef_pair_2d(n, 0.5, rng)
+    bs, ni, nm = slepc_block_policy(nev)
+    return CCSProblem(f"cfg4_{i}_2d{n}", A, B, nev, bs, ni, nm)
+
+
+def config4_batch(count: int = 512, seed: int = SEED_BASE) -> list:
+    return [config4_problem(i, seed) for i in range(count)]
+
+
+def config5_stream(count: int, qps: float, seed: int = SEED_BASE):
+    """A served request stream: Poisson arrivals at `qps`; 80 % drawn as
+    in config4_problem, 20 % 3D 7-point constant-coefficient problems with
+    n ~ U[48, 96] and nev ~ U{32..128}.  Returns [(arrival_s, problem)]."""
+    rng = np.random.default_rng([seed, 5])
+    t = 0.0
+    out = []
+    for k in range(count):
+        t += float(rng.exponential(1.0 / qps))
+        if rng.random() < 0.8:
+            prob = config4_problem(int(rng.integers(0, 1 << 30)), seed)
+        else:
+            n = int(rng.integers(48, 96 + 1))
+            nev = int(rng.integers(32, 128 + 1))
+            prob = stencil_problem(n, 3, nev, name=f"cfg5_3d{n}_nev{nev}")
+        out.append((t, prob))
+    return out

@@ -173,6 +173,7 @@ def solve(A, B, nev: int, block_size: int, nev_init: int, nev_max: int,
     lib.HIPGCG_Destroy(mv_ws, None, None, nev_max, block_size, ops)
     t.destroy_multivec(evec, nev_max)
     lib.HIPGCG_Free(ops)
+    lib.OPS_HIP_Clear(ops)
     return SolveResult(ev, nconv.value, int(stats[0]), secs, int(stats[2]), int(stats[3]),
                        dict(zip(PHASES, list(times))), vecs, cq[0].value, cq[1].value)
 

@@ -34,6 +34,10 @@ extern "C" {
  * implementation of the semantics of DefaultMultiVecQtAP
  * (src/ops_multi_vec.c:351-411), including op(A)*P left in mv_ws. */
 void OPS_HIP_Set(OPS *ops);
+/* Frees the host dense table OPS_HIP_Set installed as ops->lapack_ops, for
+ * callers that own the table struct itself (the reference's OPS_Destroy,
+ * src/ops.c:140-147, frees it recursively instead). */
+void OPS_HIP_Clear(OPS *ops);
 
 /* Host small-dense table over the LAPACKVEC layout (gcge_dense_mv);
  * replaces OPS_LAPACK_Set (app/app_lapack.c:970-1014). */
@@ -89,21 +93,33 @@ void HIP_MultiVecInfo(void **mv, int64_t *nrows, int64_t *ncols, int64_t *ldd);
  *                        (app/app_lapack.c:322-333), so a run started
  *                        after srand(s) sees the same initial block;
  *   GCGE_HIP_RAND_DEVICE U[0,1) from a counter-based generator on the
- *                        device (SplitMix64 of seed and element index). */
+ *                        device (SplitMix64 of seed and element index).
+ * The setting and the generator's position belong to the calling thread;
+ * each call restarts the sequence. */
 #define GCGE_HIP_RAND_LIBC 0
 #define GCGE_HIP_RAND_DEVICE 1
 void gcge_hip_set_random_mode(int mode, uint64_t seed);
 
-/* Waits for all work queued by this library. */
+/* Each host thread that calls into the library gets its own context: a
+ * device stream, staging buffers and a rocBLAS handle.  Tables driven
+ * from different threads therefore run concurrently on the device (a
+ * multivector or matrix may be used by any thread, one at a time).
+ *
+ * Waits for all work the calling thread queued. */
 void gcge_hip_synchronize(void);
+/* Releases the calling thread's context (stream, buffers, handle); the
+ * next call from the thread creates a new one.  Worker threads call it
+ * before they exit. */
+void gcge_hip_thread_release(void);
 /* Copies the last error message (empty if none) into buf; returns its
  * code (0 = none). */
 int gcge_hip_last_error(char *buf, int len);
 /* Number of the visible device the library uses. */
 int gcge_hip_device(void);
 
-/* Kernel timing: when enabled, HIP events are recorded on the library's
- * stream around each launch of the kernel classes below, and read back
+/* Kernel timing of the calling thread's launches: when enabled, HIP
+ * events are recorded on its stream around each launch of the kernel
+ * classes below, and read back
  * with their algorithmic bytes and flops (bytes: every operand read once
  * and every result written once).
  *   kind 0: FP64 SpMM (MatDotMultiVec), 1: FP32 shifted SpMM of the
@@ -149,7 +165,8 @@ void HIPGCG_Create(void *A, int nevMax, int block_size, int nevInit, void ***mv_
                    double **dbl_ws, int **int_ws, OPS *ops);
 void HIPGCG_Destroy(void ***mv_ws, double *dbl_ws, int *int_ws, int nevMax,
                     int block_size, OPS *ops);
-/* Releases the solver state stored in ops->eigen_solver_workspace. */
+/* Releases the solver state stored in ops->eigen_solver_workspace and the
+ * device BlockPCG workspace when one is installed. */
 void HIPGCG_Free(OPS *ops);
 /* Results of the last solve: stats[0] numIter, stats[1] nevConv,
  * stats[2] total BlockPCG iterations, stats[3] linear solves;
@@ -193,6 +210,51 @@ void HIPGCG_GetOrthStats(OPS *ops, int64_t *cholqr_blocks, int64_t *mgs_fallback
  * sigma is the fixed shift of A + sigma B (0 when autoshift is off). */
 void HIPGCG_MultiLinearSolverSetup_BlockPCG32(int max_iter, double rate, double tol,
                                               void *B, double sigma, OPS *ops);
+/* Frees the device BlockPCG's workspace (four FP32 blocks of the matrix
+ * order times the block width) and clears the slot; no counterpart in the
+ * reference, whose BlockPCG workspace is a static (src/ops_lin_sol.c:
+ * 439-465). */
+void HIPGCG_MultiLinearSolverFree_BlockPCG32(OPS *ops);
+
+/* ------------------------------------------------------------------ */
+/* Solve service: concurrent requests on one GPU                       */
+/* ------------------------------------------------------------------ */
+/* No counterpart in the reference (one problem per process there).  A
+ * pool of worker threads, each on its own device stream, takes requests
+ * in arrival order as soon as a worker is free and the request's device
+ * footprint (gcge_request_footprint) fits in what the HBM budget has
+ * left; each request runs this library's driver over its own table. */
+#define GCGE_MODE_FP64 0
+#define GCGE_MODE_MIXED 1
+typedef struct {
+    const gcge_ccs_mat *A, *B; /* host CCS; valid until the request completes */
+    int nev, block_size, nev_init, nev_max, max_iter;
+    int mode;  /* GCGE_MODE_FP64 (autoshift per `shift`, MGS) or
+                  GCGE_MODE_MIXED (FP32 BlockPCG, CholQR2, no autoshift) */
+    int shift;
+    double tol_abs, tol_rel, gap_min;
+    uint64_t seed; /* initial block from the device generator */
+} gcge_request;
+typedef struct {
+    int status;     /* 1 pending, 0 done, -1 matrix rejected, -2 over budget */
+    int nev_conv, num_iter;
+    int concurrent; /* requests on the device when this one started, itself included */
+    double queue_seconds, setup_seconds, solve_seconds, total_seconds;
+    double *eval;   /* caller's array of nev_max doubles, or NULL */
+    double *evec;   /* caller's N x nev_max column-major array, or NULL */
+} gcge_result;
+/* Device bytes a request holds while it runs. */
+int64_t gcge_request_footprint(const gcge_request *req);
+void *gcge_service_create(int workers, int64_t hbm_budget_bytes);
+/* Queues a request; returns its ticket, or -1 (res->status says why).  The
+ * service writes into *res until gcge_service_wait returns. */
+int64_t gcge_service_submit(void *svc, const gcge_request *req, gcge_result *res);
+/* Blocks until the request is done; returns its status. */
+int gcge_service_wait(void *svc, int64_t ticket);
+void gcge_service_stats(void *svc, int64_t *completed, int *max_concurrent,
+                        int64_t *hbm_in_use);
+/* Finishes the queued requests and joins the workers. */
+void gcge_service_destroy(void *svc);
 
 #ifdef __cplusplus
 }

@@ -94,3 +94,33 @@ def test_layout_matches_reference_header(tmp_path):
     ours = run_probe(tmp_path, '"gcge_ops.h"', [f"-I{INCLUDE}"], "gcge_ccs_mat",
                      "gcge_dense_mv", "ours2")
     assert theirs == ours
+
+
+SERVICE_PROBE = r"""
+#include <stdio.h>
+#include <stddef.h>
+#include "gcge_hip.h"
+#define R(x) printf("req %s %zu\n", #x, offsetof(gcge_request, x));
+#define S(x) printf("res %s %zu\n", #x, offsetof(gcge_result, x));
+int main(void) {
+  FIELDS
+  printf("req sizeof %zu\nres sizeof %zu\n", sizeof(gcge_request), sizeof(gcge_result));
+  return 0;
+}
+"""
+
+
+def test_service_structs_match_ctypes(tmp_path):
+    fields = " ".join([f"R({n})" for n, _ in abi.Request._fields_] +
+                      [f"S({n})" for n, _ in abi.Result._fields_])
+    src = tmp_path / "svc.c"
+    src.write_text(SERVICE_PROBE.replace("FIELDS", fields))
+    exe = tmp_path / "svc"
+    subprocess.run(["gcc", "-std=gnu99", f"-I{INCLUDE}", str(src), "-o", str(exe)], check=True,
+                   capture_output=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split()
+    got = {(out[i], out[i + 1]): int(out[i + 2]) for i in range(0, len(out), 3)}
+    for tag, cls in (("req", abi.Request), ("res", abi.Result)):
+        for n, _ in cls._fields_:
+            assert got[(tag, n)] == getattr(cls, n).offset, (tag, n)
+        assert got[(tag, "sizeof")] == C.sizeof(cls)

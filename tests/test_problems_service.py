@@ -1,0 +1,77 @@
+"""Problem generators of the batched and served configurations, and the
+service's admission footprint (host-only: no device calls)."""
+import ctypes as C
+
+import numpy as np
+import scipy.linalg as sla
+
+from gcge_amd import abi, problems as P
+from gcge_amd.service import Service
+
+
+def test_varcoef_reduces_to_constant_coefficients():
+    A, B = P.varcoef_pair_2d(12, 0.0, np.random.default_rng(1))
+    A0, B0 = P.stencil_pair(12, 2)
+    assert abs(A - A0).max() == 0.0 and abs(B - B0).max() == 0.0
+
+
+def test_varcoef_is_spd_with_harmonic_edges():
+    rng = np.random.default_rng(7)
+    A, B = P.varcoef_pair_2d(9, 0.5, rng)
+    D = A.toarray()
+    assert np.array_equal(D, D.T)
+    assert np.linalg.eigvalsh(D).min() > 0
+    # an interior edge carries the harmonic mean of its nodes' coefficients
+    kappa = np.exp(0.5 * np.random.default_rng(7).standard_normal(81))
+    i, j = 4 + 9 * 4, 5 + 9 * 4
+    assert np.isclose(-D[i, j], 2 * kappa[i] * kappa[j] / (kappa[i] + kappa[j]), rtol=1e-14)
+    # row sums: only boundary nodes keep a positive excess (the ghost terms)
+    rs = D.sum(axis=1).reshape(9, 9)
+    assert np.allclose(rs[1:-1, 1:-1], 0.0, atol=1e-12)
+    assert np.all(rs[0, :] > 0)
+
+
+def test_config4_distribution():
+    for i in range(6):
+        p = P.config4_problem(i)
+        n = int(round(np.sqrt(p.n)))
+        assert n * n == p.n and 45 <= n <= 256
+        assert 8 <= p.nev <= 64
+        bs, ni, nm = P.slepc_block_policy(p.nev)
+        assert (p.block_size, p.nev_init, p.nev_max) == (bs, ni, nm)
+    # deterministic in (seed, index)
+    a, b = P.config4_problem(3), P.config4_problem(3)
+    assert a.n == b.n and abs(a.A - b.A).max() == 0.0
+
+
+def test_config5_stream_mix():
+    s = P.config5_stream(200, 4.0)
+    t = np.array([x[0] for x in s])
+    assert np.all(np.diff(t) > 0)
+    gaps = np.diff(np.concatenate([[0.0], t]))
+    assert 0.15 < gaps.mean() < 0.35               # Poisson at 4 per second
+    n3 = sum(1 for _, p in s if p.name.startswith("cfg5_3d"))
+    assert 20 <= n3 <= 60                          # about 20 % 3D problems
+
+
+def test_footprint_matches_the_workspace_formula():
+    """Multivectors 8 N (2 nevMax + 5 bs) plus the matrices, within the
+    stated slack (matrix padding, partials, projected matrices)."""
+    prob = P.config2()
+    req, _keep = Service.request(prob, mode="fp64")
+    lib = abi.load()
+    fp = lib.gcge_request_footprint(C.byref(req))
+    mv = 8 * prob.n * (2 * prob.nev_max + 5 * prob.block_size)
+    mats = 2 * (12 * prob.A.nnz + 4 * (prob.n + 1))
+    assert mv + mats <= fp <= 1.05 * (mv + mats) + (64 << 20)
+    req, _keep = Service.request(prob, mode="mixed")
+    fpm = lib.gcge_request_footprint(C.byref(req))
+    assert fpm - fp >= 4 * 4 * prob.n * prob.block_size   # FP32 BlockPCG blocks
+
+
+def test_dense_oracle_for_varcoef():
+    """The dense generalized eigensolver the GPU tests compare with agrees
+    with the constant-coefficient closed form (a check of the harness)."""
+    A, B = P.varcoef_pair_2d(10, 0.0, np.random.default_rng(0))
+    w = sla.eigh(A.toarray(), B.toarray(), eigvals_only=True, subset_by_index=[0, 9])
+    assert np.allclose(w, P.analytic_eigenvalues(10, 2, 10), rtol=1e-12)
