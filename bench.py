@@ -279,6 +279,72 @@ def cpu_baseline(prob, gpu_iters, nev_conv, iters_sample):
             "seconds_extrapolated": t_full}, None
 
 
+def served(args):
+    """The batched (cfg4) and served (cfg5) configurations through the
+    solve service: one JSON line with throughput and latency."""
+    from gcge_amd import problems as P
+    from gcge_amd.service import Service
+    import scipy.sparse.linalg  # noqa: F401  (residual checks below)
+    t0 = time.perf_counter()
+    if args.config == "cfg4":
+        items = [(0.0, p) for p in P.config4_batch(args.count)]
+        what = (f"cfg4: {args.count} 2D variable-coefficient 5-point problems, N in "
+                f"[2048, 65536], nev in 8..64, submitted together")
+    else:
+        items = P.config5_stream(args.count, args.qps)
+        what = (f"cfg5: {args.count} requests, Poisson arrivals at {args.qps} per s, 80 % "
+                f"cfg4-like, 20 % 3D 7-point 48^3..96^3 with nev 32..128")
+    log(f"{args.config}: generated {len(items)} problems in {time.perf_counter() - t0:.1f}s")
+    check_every = max(1, len(items) // 8)
+    res = [None] * len(items)
+    with Service(workers=args.workers, hbm_budget=args.hbm_budget) as svc:
+        # warm the workers' kernels and handles on a small problem each
+        warm = [svc.submit(P.stencil_problem(24, 2, 8), seed=1) for _ in range(args.workers)]
+        for t in warm:
+            svc.wait(t)
+        tickets = []
+        start = time.perf_counter()
+        for k, (t_arr, prob) in enumerate(items):
+            delay = start + t_arr - time.perf_counter()
+            if delay > 0:
+                time.sleep(delay)
+            tickets.append(svc.submit(prob, mode="mixed", seed=1000 + k,
+                                      want_vectors=(k % check_every == 0)))
+        for k, t in enumerate(tickets):
+            res[k] = svc.wait(t)
+        wall = time.perf_counter() - start
+        st = svc.stats()
+    lat = np.array([r.total_seconds for r in res])
+    conv = sum(r.nev_conv for r in res)
+    short = sum(1 for (_, p), r in zip(items, res) if r.nev_conv < p.nev)
+    worst = 0.0
+    for k, ((_, p), r) in enumerate(zip(items, res)):
+        if r.evec is not None:
+            R = p.A @ r.evec - (p.B @ r.evec) * r.eval[None, :]
+            rr = np.linalg.norm(R, axis=0) / np.abs(r.eval)
+            worst = max(worst, float(rr.max()))
+    line = {
+        "metric": "converged eigenpairs/s at 1e-8 residual, " +
+                  ("batched" if args.config == "cfg4" else "served") +
+                  "; p50/p99 request latency",
+        "value": conv / wall, "unit": "eigenpairs/s", "n_gpus": 1, "steps": 1, "warmup": 1,
+        "ms_per_step": 1e3 * wall, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64+f32", "data": "synthetic",
+        "config": {"workload": what, "path": "mixed", "workers": args.workers,
+                   "hbm_budget_bytes": args.hbm_budget, "parallelism": "single GPU"},
+        "latency_ms": {"p50": 1e3 * float(np.percentile(lat, 50)),
+                       "p99": 1e3 * float(np.percentile(lat, 99))},
+        "requests": len(items), "eigenpairs_converged": conv,
+        "requests_short_of_nev": short, "max_concurrent": st["max_concurrent"],
+        "max_rel_residual_sampled": worst,
+        "queue_s_p50": float(np.percentile([r.queue_seconds for r in res], 50)),
+        "solve_s_p50": float(np.percentile([r.solve_seconds for r in res], 50)),
+        "roofline": None, "cpu_baseline": None,
+        "cpu_baseline_note": "not measured for the batched/served configurations",
+    }
+    print(json.dumps(line))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -289,6 +355,10 @@ def main():
     ap.add_argument("--cpu-baseline", default="on", choices=("on", "off"))
     ap.add_argument("--cpu-iters", type=int, default=1)
     ap.add_argument("--copy-probe", action="store_true")
+    ap.add_argument("--count", type=int, default=512, help="cfg4/cfg5 requests")
+    ap.add_argument("--qps", type=float, default=20.0, help="cfg5 arrival rate")
+    ap.add_argument("--workers", type=int, default=8, help="cfg4/cfg5 service workers")
+    ap.add_argument("--hbm-budget", type=float, default=200e9)
     ap.add_argument("--rocprof-summary", metavar="DIR")
     ap.add_argument("--pmc-traffic", nargs=5,
                     metavar=("FETCH", "WRITE", "COPY_FETCH", "COPY_WRITE", "BENCH_JSON"))
@@ -306,6 +376,9 @@ def main():
 
     if args.copy_probe:
         copy_probe()
+        return
+    if args.config in ("cfg4", "cfg5"):
+        served(args)
         return
     from gcge_amd import abi, problems as P, solver as S
     abi.load().gcge_host_set_print_target(1)  # monitor lines to stderr
