@@ -54,6 +54,8 @@ def make_problem(name: str):
         return P.config2(), 128, 3
     if name == "cfg1":
         return P.config1(), 128, 2
+    if name == "cfg3":
+        return P.config3(), 160, None      # no closed form
     if name.startswith("3d_"):
         n = int(name[3:])
         return P.stencil_problem(n, 3, 100, 120, 220, 220, name=f"3d_{n}"), n, 3
@@ -92,9 +94,12 @@ def run_path(prob, n, dim, mode, steps, warmup, mats, want_check=True):
     k = res.nev_conv
     out = {"seconds": secs, "num_iter": res.num_iter, "nev_conv": k,
            "bpcg_iters": res.bpcg_iters, "phases": res.phases, "kernels": prof}
+    out["eval"] = [float(x) for x in res.eval[:k]]
     if want_check and res.evec is not None:
-        exact = P.analytic_eigenvalues(n, dim, k)
-        out["max_rel_eig_err_vs_analytic"] = float(np.max(np.abs(res.eval[:k] - exact) / exact))
+        if dim is not None:
+            exact = P.analytic_eigenvalues(n, dim, k)
+            out["max_rel_eig_err_vs_analytic"] = float(np.max(np.abs(res.eval[:k] - exact) /
+                                                              exact))
         X = res.evec[:, :k]
         lam = res.eval[:k]
         R = prob.A @ X - (prob.B @ X) * lam[None, :]
@@ -432,7 +437,7 @@ def main():
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f64+f32" if head == "mixed" else "f64", "data": "synthetic",
         "config": {"workload": (f"{prob.name}: generalized eigenproblem, N={prob.n}, "
-                                f"nnz={prob.A.nnz} per matrix, nev={prob.nev}, "
+                                f"nnz(A)={prob.A.nnz}, nnz(B)={prob.B.nnz}, nev={prob.nev}, "
                                 f"block {prob.block_size}, nevInit=nevMax={prob.nev_max}, "
                                 f"tol 1e-8 rel"),
                    "path": head, "parallelism": "single GPU"},
@@ -443,7 +448,8 @@ def main():
                       "nev_conv": r["nev_conv"], "bpcg_iters": r["bpcg_iters"],
                       "max_rel_eig_err_vs_analytic": r.get("max_rel_eig_err_vs_analytic"),
                       "max_rel_residual": r.get("max_rel_residual"),
-                      "phases_s": r["phases"], "kernels": r["kernels"]}
+                      "phases_s": r["phases"], "kernels": r["kernels"],
+                      "eval_first_last": r.get("eval_first_last")}
                   for m, r in results.items()},
         "upload_s": t_upload,
         "roofline": roof,
@@ -452,6 +458,9 @@ def main():
     if "fp64" in results and "mixed" in results:
         line["speedup_mixed_vs_fp64"] = (np.mean(results["fp64"]["seconds"]) /
                                          np.mean(results["mixed"]["seconds"]))
+        a, b = (np.array(results[m]["eval"]) for m in ("fp64", "mixed"))
+        k = min(len(a), len(b))
+        line["max_rel_eig_diff_fp64_vs_mixed"] = float(np.max(np.abs(a[:k] - b[:k]) / a[:k]))
     if cpu is None:
         line["cpu_baseline_note"] = why
     print(json.dumps(line))

@@ -109,20 +109,23 @@ __global__ __launch_bounds__(kWG) void k_spmm_sell_wide(
     }
 }
 
-template <typename V, int T>
+// Column tiles narrow as rows widen (W entries x T columns of gathers in
+// flight per thread), so registers stay near the same budget: 4 columns
+// for rows of up to 8 entries, 2 up to 16, 1 up to 32.
+template <typename V>
 void launch_spmm(const SellMat &A, const V *val, const V *X, int64_t ldx, V *Y,
                  int64_t ldy, int ncols, hipStream_t s)
 {
     if (ncols <= 0 || A.nrows == 0) return;
     const unsigned g = grid1d(A.nrows);
     if (A.max_width <= 8)
-        k_spmm_sell<V, 8, T><<<g, kWG, 0, s>>>(A.nrows, A.slice_ptr, A.col, val, X, ldx, Y,
+        k_spmm_sell<V, 8, 4><<<g, kWG, 0, s>>>(A.nrows, A.slice_ptr, A.col, val, X, ldx, Y,
                                                ldy, ncols);
     else if (A.max_width <= 16)
-        k_spmm_sell<V, 16, T><<<g, kWG, 0, s>>>(A.nrows, A.slice_ptr, A.col, val, X, ldx, Y,
+        k_spmm_sell<V, 16, 2><<<g, kWG, 0, s>>>(A.nrows, A.slice_ptr, A.col, val, X, ldx, Y,
                                                 ldy, ncols);
     else if (A.max_width <= 32)
-        k_spmm_sell<V, 32, T><<<g, kWG, 0, s>>>(A.nrows, A.slice_ptr, A.col, val, X, ldx, Y,
+        k_spmm_sell<V, 32, 1><<<g, kWG, 0, s>>>(A.nrows, A.slice_ptr, A.col, val, X, ldx, Y,
                                                 ldy, ncols);
     else
         k_spmm_sell_wide<V><<<g, kWG, 0, s>>>(A.nrows, A.slice_ptr, A.col, val, X, ldx, Y,
@@ -398,7 +401,7 @@ __global__ __launch_bounds__(kWG) void k_to_f64(int64_t nrows, const float *__re
 void spmm(const SellMat &A, const double *X, int64_t ldx, double *Y, int64_t ldy, int ncols,
           hipStream_t s)
 {
-    launch_spmm<double, 4>(A, A.val, X, ldx, Y, ldy, ncols, s);
+    launch_spmm<double>(A, A.val, X, ldx, Y, ldy, ncols, s);
 }
 
 void axpby(int64_t nrows, int ncols, double alpha, const double *X, int64_t ldx, double beta,

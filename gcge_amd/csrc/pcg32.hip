@@ -190,7 +190,9 @@ __global__ __launch_bounds__(kWG) void k_zero_tiles(int64_t n, float *__restrict
 // Shifted SpMM on the listed tiles with the p.w partial sums fused:
 // w = (A + sigma B) p for the 8 columns of tile tiles[k], k < ntact;
 // partial[(tile*8 + t) * nblk + block] = the block's sum of p * w.
-template <int W>
+// TWO: B is a matrix of its own (a shift by a B other than A); otherwise
+// the operator is A (+ sigma A) and only A's entries are held in registers.
+template <int W, bool TWO>
 __global__ __launch_bounds__(kWG) void k_spmm_pw(
     int64_t n, const int64_t *__restrict__ slice_ptr, const int *__restrict__ colA,
     const float *__restrict__ valA, const int *__restrict__ colB,
@@ -206,10 +208,10 @@ __global__ __launch_bounds__(kWG) void k_spmm_pw(
     const int64_t blk = blockIdx.x;
     const int64_t row = blk * kWG + threadIdx.x;
     const bool live = row < n;
-    int ca[W], cb[W];
-    float va[W], vb[W];
+    constexpr int WB = TWO ? W : 1;
+    int ca[W], cb[WB];
+    float va[W], vb[WB];
     int w = 0;
-    const bool same = (colA == colB);
     if (live) {
         const int64_t s = row >> 6;
         const int lane = (int)(row & 63);
@@ -220,16 +222,20 @@ __global__ __launch_bounds__(kWG) void k_spmm_pw(
             if (j < w) {
                 const int64_t e = base + (int64_t)j * 64 + lane;
                 ca[j] = colA[e];
-                va[j] = valA[e];
-                vb[j] = sigma * valB[e];
-                cb[j] = same ? ca[j] : colB[e];
-                if (same) {
-                    va[j] += vb[j];
-                    vb[j] = 0.f;
+                if constexpr (TWO) {
+                    va[j] = valA[e];
+                    vb[j] = sigma * valB[e];
+                    cb[j] = colB[e];
+                } else {
+                    va[j] = valB ? valA[e] + sigma * valB[e] : valA[e];
                 }
             } else {
-                ca[j] = cb[j] = (int)row;
-                va[j] = vb[j] = 0.f;
+                ca[j] = (int)row;
+                va[j] = 0.f;
+                if constexpr (TWO) {
+                    cb[j] = (int)row;
+                    vb[j] = 0.f;
+                }
             }
         }
     }
@@ -250,7 +256,7 @@ __global__ __launch_bounds__(kWG) void k_spmm_pw(
                     const f8 pa = load8(pt + (int64_t)ca[j] * kC);
 #pragma unroll
                     for (int t = 0; t < kC; ++t) acc.v[t] += va[j] * pa.v[t];
-                    if (!same) {
+                    if constexpr (TWO) {
                         const f8 pb = load8(pt + (int64_t)cb[j] * kC);
 #pragma unroll
                         for (int t = 0; t < kC; ++t) acc.v[t] += vb[j] * pb.v[t];
@@ -543,14 +549,19 @@ void pcg32_solve(void *matA, void **mv_b, void **mv_x, int *start, int *end, OPS
         hipEvent_t ev = c.prof.begin(PROF_SPMM32,
                                      mat_bytes + 2.0 * 4.0 * kC * (double)ntact * n,
                                      2.0 * (double)A->sell.nnz_true * kC * ntact, st);
-        if (A->sell.max_width <= 8)
-            k_spmm_pw<8><<<nblk_row, kWG, 0, st>>>(n, A->sell.slice_ptr, A->sell.col,
-                                                   A->sell.val32, Bs->col, Bs->val32, sigma,
-                                                   S.tiles, S.ntact, S.p, S.w, S.partial);
-        else
-            k_spmm_pw<32><<<nblk_row, kWG, 0, st>>>(n, A->sell.slice_ptr, A->sell.col,
-                                                    A->sell.val32, Bs->col, Bs->val32, sigma,
-                                                    S.tiles, S.ntact, S.p, S.w, S.partial);
+        // B's entries enter separately only for a shift by a B other than A
+        const bool two = sigma != 0.f && Bs->col != A->sell.col;
+        const float *vB = sigma != 0.f ? Bs->val32 : nullptr;
+#define SPMM_PW(W, TWO)                                                                       \
+    k_spmm_pw<W, TWO><<<nblk_row, kWG, 0, st>>>(n, A->sell.slice_ptr, A->sell.col,            \
+                                                A->sell.val32, Bs->col, vB, sigma, S.tiles,    \
+                                                S.ntact, S.p, S.w, S.partial)
+        if (A->sell.max_width <= 8) {
+            if (two) SPMM_PW(8, true); else SPMM_PW(8, false);
+        } else {
+            if (two) SPMM_PW(32, true); else SPMM_PW(32, false);
+        }
+#undef SPMM_PW
         c.prof.end(ev, st);
         k_finish_pw<<<ntact * kC, 64, 0, st>>>(S.tiles, S.ntact, (int)nblk_row, S.partial, S.rho,
                                                S.active, S.alpha);

@@ -218,3 +218,79 @@ def config5_stream(count: int, qps: float, seed: int = SEED_BASE):
             prob = stencil_problem(n, 3, nev, name=f"cfg5_3d{n}_nev{nev}")
         out.append((t, prob))
     return out
+
+
+def varcoef27_pair(n: int, sigma: float, rng: np.random.Generator,
+                   delta_rel: float = 1e-8):
+    """Variable-coefficient diffusion on an n^3 interior grid with the
+    27-point stencil, Dirichlet boundary, index x + n*y + n^2*z.
+
+    Node coefficients kappa = exp(sigma * z), z ~ N(0, 1).  The neighbour
+    at offset d (|d|_inf = 1) couples with weight g(d) * 2 kappa_i kappa_j /
+    (kappa_i + kappa_j), g(d) = 1 / |d|^2 (faces 1, edges 1/2, corners
+    1/3); A_ij = -weight, A_ii = the sum of the weights at i plus, for
+    every neighbour outside the grid, g(d) * kappa_i, plus delta = delta_rel
+    * max diagonal.  B = diag(U[0.5, 1.5]).  A is symmetric bit for bit and
+    strictly diagonally dominant, hence SPD; nnz(A) = (3n - 2)^3.
+
+    Built column by column in CCS directly: the 27 offsets in (dz, dy, dx)
+    order are the columns of a row in ascending order."""
+    N = n ** 3
+    kappa = np.exp(sigma * rng.standard_normal(N))
+    bdiag = rng.uniform(0.5, 1.5, N)
+    iz, iy, ix = (a.ravel() for a in np.indices((n, n, n), dtype=np.int32))
+    offs = [(dz, dy, dx) for dz in (-1, 0, 1) for dy in (-1, 0, 1) for dx in (-1, 0, 1)]
+
+    def valid(dz, dy, dx):
+        m = np.ones(N, dtype=bool)
+        for a, d in ((iz, dz), (iy, dy), (ix, dx)):
+            if d < 0:
+                m &= a > 0
+            elif d > 0:
+                m &= a < n - 1
+        return m
+
+    counts = np.zeros(N, dtype=np.int64)
+    for o in offs:
+        counts += valid(*o)
+    indptr = np.zeros(N + 1, dtype=np.int64)
+    np.cumsum(counts, out=indptr[1:])
+    nnz = int(indptr[-1])
+    indices = np.empty(nnz, dtype=np.int32)
+    data = np.empty(nnz, dtype=np.float64)
+    diag = np.zeros(N)
+    pos = indptr[:-1].copy()
+    pos_diag = None
+    for dz, dy, dx in offs:
+        m = valid(dz, dy, dx)
+        r = np.flatnonzero(m)
+        if (dz, dy, dx) == (0, 0, 0):
+            pos_diag = pos.copy()
+            pos += 1
+            continue
+        g = 1.0 / (dz * dz + dy * dy + dx * dx)
+        c = r + (dz * n * n + dy * n + dx)
+        w = g * (2.0 * kappa[r] * kappa[c] / (kappa[r] + kappa[c]))
+        indices[pos[r]] = c
+        data[pos[r]] = -w
+        pos[r] += 1
+        diag[r] += w
+        out = ~m
+        diag[out] += g * kappa[out]
+    diag += delta_rel * diag.max()
+    indices[pos_diag] = np.arange(N, dtype=np.int32)
+    data[pos_diag] = diag
+    A = sp.csc_matrix((data, indices, indptr.astype(np.int32)), shape=(N, N))
+    B = sp.diags(bdiag).tocsc()
+    B.sort_indices()
+    return A, B
+
+
+def config3(n: int = 160, nev: int = 200, seed: int = SEED_BASE) -> CCSProblem:
+    """27-point variable-coefficient diffusion (lognormal sigma = 1) on n^3,
+    diagonal B, nev = 200 with the SLEPc block policy (bs 40, nevInit 120,
+    nevMax 320), tol 1e-8."""
+    rng = np.random.default_rng([seed, 3])
+    A, B = varcoef27_pair(n, 1.0, rng)
+    bs, ni, nm = slepc_block_policy(nev)
+    return CCSProblem(f"cfg3_27pt_{n}", A, B, nev, bs, ni, nm)

@@ -33,14 +33,32 @@ CASES = [
     ("3d_12", 12, 3, 30, 1, ("ccs",)),
     ("3d_12", 12, 3, 30, 0, ("ccs",)),
     ("3d_16", 16, 3, 40, 0, ("ccs",)),
+    # variable coefficients: dim 2 is the 5-point operator of the batched
+    # configuration (sigma 0.5), dim 27 the 27-point one of config 3 (sigma 1)
+    ("vc2d_40", 40, "vc2", 16, 1, ("ccs",)),
+    ("vc2d_40", 40, "vc2", 16, 0, ("ccs",)),
+    ("vc27_16", 16, "vc27", 20, 1, ("ccs",)),
+    ("vc27_16", 16, "vc27", 20, 0, ("ccs",)),
 ]
+
+
+def problem(name, n, dim, nev):
+    """The seeded problem of a case (also used by the tests)."""
+    if dim == "vc2":
+        A, B = P.varcoef_pair_2d(n, 0.5, np.random.default_rng([P.SEED_BASE, 2000 + n]))
+    elif dim == "vc27":
+        A, B = P.varcoef27_pair(n, 1.0, np.random.default_rng([P.SEED_BASE, 2700 + n]))
+    else:
+        return P.stencil_problem(n, dim, nev)
+    bs, ni, nm = P.slepc_block_policy(nev)
+    return P.CCSProblem(name, A, B, nev, bs, ni, nm)
 
 
 def main():
     ref = RefLib()
     out = []
     for name, n, dim, nev, shift, apps in CASES:
-        prob = P.stencil_problem(n, dim, nev)
+        prob = problem(name, n, dim, nev)
         for app in apps:
             if app == "ccs":
                 r = ref.solve_ccs(prob, shift=shift)

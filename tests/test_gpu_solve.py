@@ -10,6 +10,7 @@
 import ctypes as C
 import json
 import os
+import sys
 
 import numpy as np
 import pytest
@@ -113,3 +114,53 @@ def test_dropin_reference_driver(hip, reflib):
     exact = P.analytic_eigenvalues(32, 2, k)
     assert np.max(np.abs(r["eval"][:k] - exact) / exact) <= 1e-10
     assert abs(r["num_iter"] - g["num_iter"]) <= 4
+
+
+def vc_problem(name, n, kind, nev):
+    sys.path.insert(0, os.path.join(os.path.dirname(__file__), "golden"))
+    from make_golden import problem
+    return problem(name, n, kind, nev)
+
+
+@pytest.mark.parametrize("name,n,kind,nev", [("vc2d_40", 40, "vc2", 16),
+                                             ("vc27_16", 16, "vc27", 20)])
+def test_fp64_variable_coefficients(hip, name, n, kind, nev):
+    """No closed form: the compiled reference's runs and dense dsygv."""
+    prob = vc_problem(name, n, kind, nev)
+    g = golden(name, 1)
+    res = S.solve_problem(prob, shift=1, want_vectors=True)
+    assert res.nev_conv >= nev
+    assert abs(res.num_iter - g["num_iter"]) <= 8
+    k = min(res.nev_conv, g["nev_conv"])
+    ref = np.array(g["eval"][:k])
+    assert np.max(np.abs(res.eval[:k] - ref) / ref) <= 1e-9
+    rr = O.relative_residuals(prob.A, prob.B, res.eval[:res.nev_conv],
+                              res.evec[:, :res.nev_conv])
+    assert rr.max() <= 1.01e-8
+
+
+def test_mixed_variable_coefficients_27pt(hip):
+    prob = vc_problem("vc27_16", 16, "vc27", 20)
+    res = S.solve_problem(prob, shift=0, mode="mixed", want_vectors=True)
+    k = res.nev_conv
+    assert k >= 20
+    ex = O.dense_generalized_eigenvalues(prob.A, prob.B, k)
+    assert np.max(np.abs(res.eval[:k] - ex) / ex) <= 1e-9
+    rr = O.relative_residuals(prob.A, prob.B, res.eval[:k], res.evec[:, :k])
+    assert rr.max() <= 1.01e-8
+
+
+def test_27pt_32cubed_fp64_and_mixed_agree(hip):
+    """The config-3 family at 32^3 (N = 32768, rows of 27 entries: the
+    32-wide SELL path): the two precisions agree and meet the residual."""
+    A, B = P.varcoef27_pair(32, 1.0, np.random.default_rng([P.SEED_BASE, 3]))
+    bs, ni, nm = P.slepc_block_policy(40)
+    prob = P.CCSProblem("cfg3_27pt_32", A, B, 40, bs, ni, nm)
+    r64 = S.solve_problem(prob, shift=1, want_vectors=True)
+    rmx = S.solve_problem(prob, shift=0, mode="mixed", want_vectors=True)
+    k = min(r64.nev_conv, rmx.nev_conv)
+    assert k >= 40
+    assert np.max(np.abs(r64.eval[:k] - rmx.eval[:k]) / r64.eval[:k]) <= 1e-9
+    for r in (r64, rmx):
+        rr = O.relative_residuals(A, B, r.eval[:r.nev_conv], r.evec[:, :r.nev_conv])
+        assert rr.max() <= 1.01e-8

@@ -75,3 +75,20 @@ def test_dense_oracle_for_varcoef():
     A, B = P.varcoef_pair_2d(10, 0.0, np.random.default_rng(0))
     w = sla.eigh(A.toarray(), B.toarray(), eigvals_only=True, subset_by_index=[0, 9])
     assert np.allclose(w, P.analytic_eigenvalues(10, 2, 10), rtol=1e-12)
+
+
+def test_varcoef27_structure():
+    A, B = P.varcoef27_pair(7, 1.0, np.random.default_rng(5))
+    D = A.toarray()
+    assert np.array_equal(D, D.T)                       # bit for bit
+    assert A.nnz == (3 * 7 - 2) ** 3 and A.has_sorted_indices
+    off = D - np.diag(np.diag(D))
+    assert np.all(np.diag(D) > -off.sum(axis=1))        # strictly dominant
+    # an interior face edge: harmonic mean, weight 1; a corner edge: 1/3
+    kappa = np.exp(np.random.default_rng(5).standard_normal(343))
+    c = 3 + 7 * 3 + 49 * 3
+    for d, g in ((1, 1.0), (1 + 7 + 49, 1.0 / 3.0)):
+        h = 2 * kappa[c] * kappa[c + d] / (kappa[c] + kappa[c + d])
+        assert np.isclose(-D[c, c + d], g * h, rtol=1e-14)
+    bd = B.diagonal()
+    assert B.nnz == 343 and bd.min() >= 0.5 and bd.max() <= 1.5
