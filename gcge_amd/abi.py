@@ -158,6 +158,7 @@ EXPORTS = {
     "HIPGCG_GetStats": (None, [OPS_p, C.POINTER(C.c_int64), c_dbl_p]),
     "HIPGCG_SetDenseEigensolver": (None, [OPS_p, C.c_int]),
     "HIPGCG_SetOrthMethod": (None, [OPS_p, C.c_int]),
+    "gcge_host_dense_eigh": (C.c_int, [C.c_int, c_dbl_p, C.c_int, c_dbl_p, c_dbl_p, C.c_int]),
     "HIPGCG_GetOrthStats": (None, [OPS_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
     "HIPGCG_MultiLinearSolverSetup_BlockPCG32": (None, [C.c_int, C.c_double, C.c_double,
                                                         C.c_void_p, C.c_double, OPS_p]),

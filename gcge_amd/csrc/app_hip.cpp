@@ -76,6 +76,11 @@ void release_ctx()
     if (c->d_scratch) HIPCHECK(hipFree(c->d_scratch));
     if (c->d_small) HIPCHECK(hipFree(c->d_small));
     if (c->h_pinned) HIPCHECK(hipHostFree(c->h_pinned));
+    for (Ctx::Staging &g : c->stage) {
+        if (g.h) HIPCHECK(hipHostFree(g.h));
+        if (g.d) HIPCHECK(hipFree(g.d));
+        if (g.ev) HIPCHECK(hipEventDestroy(g.ev));
+    }
     for (hipEvent_t e : c->prof.pool) HIPCHECK(hipEventDestroy(e));
     HIPCHECK(hipStreamDestroy(c->stream));
     delete c;
@@ -121,6 +126,32 @@ double *Ctx::pinned(int64_t ndouble)
 }
 
 void Ctx::sync() { HIPCHECK(hipStreamSynchronize(stream)); }
+
+Ctx::Staging &Ctx::staging(int64_t ndouble)
+{
+    Staging &g = stage[stage_next];
+    stage_next = (stage_next + 1) % kStage;
+    if (g.pending) {
+        HIPCHECK(hipEventSynchronize(g.ev));
+        g.pending = false;
+    }
+    if (ndouble > g.cap) {
+        if (g.h) HIPCHECK(hipHostFree(g.h));
+        if (g.d) HIPCHECK(hipFree(g.d));
+        g.cap = ndouble + 4096;
+        HIPCHECK(hipHostMalloc(reinterpret_cast<void **>(&g.h), g.cap * sizeof(double),
+                               hipHostMallocDefault));
+        HIPCHECK(hipMalloc(reinterpret_cast<void **>(&g.d), g.cap * sizeof(double)));
+    }
+    if (g.ev == nullptr) HIPCHECK(hipEventCreateWithFlags(&g.ev, hipEventDisableTiming));
+    return g;
+}
+
+void Ctx::staged(Staging &g)
+{
+    HIPCHECK(hipEventRecord(g.ev, stream));
+    g.pending = true;
+}
 
 // ---------------------------------------------------------------- profiler
 hipEvent_t Profiler::begin(int kind, double b, double f, hipStream_t s)
@@ -374,12 +405,12 @@ void h_MultiVecLinearComb(void **x, void **y, int, int *start, int *end, double 
         if (incb == 0) {
             if (*beta != 1.0) scale_cols(Y->nrows, nc, nullptr, *beta, ys, Y->ldd, c.stream);
         } else {
-            double *h = c.pinned(nc);
-            for (int k = 0; k < nc; ++k) h[k] = beta[(int64_t)k * incb];
-            double *d = c.small(nc);
-            HIPCHECK(hipMemcpyAsync(d, h, sizeof(double) * nc, hipMemcpyHostToDevice, c.stream));
-            scale_cols(Y->nrows, nc, d, 0.0, ys, Y->ldd, c.stream);
-            c.sync();  // the pinned staging buffer is reused by the next call
+            Ctx::Staging &g = c.staging(nc);
+            for (int k = 0; k < nc; ++k) g.h[k] = beta[(int64_t)k * incb];
+            HIPCHECK(hipMemcpyAsync(g.d, g.h, sizeof(double) * nc, hipMemcpyHostToDevice,
+                                    c.stream));
+            scale_cols(Y->nrows, nc, g.d, 0.0, ys, Y->ldd, c.stream);
+            c.staged(g);
         }
         return;
     }
@@ -389,7 +420,8 @@ void h_MultiVecLinearComb(void **x, void **y, int, int *start, int *end, double 
     // stage coef transposed and k-major (Ct[i * nc + k] = coef[i + k * ldc]),
     // the operand layout of the matrix-core kernel, and beta, in one upload
     const int64_t ncoef = (int64_t)nr * nc;
-    double *h = c.pinned(ncoef + nc);
+    Ctx::Staging &g = c.staging(ncoef + nc);
+    double *h = g.h;
     for (int k = 0; k < nc; ++k) {
         const double *src = coef + (int64_t)ldc * k;
         for (int i = 0; i < nr; ++i) h[(int64_t)i * nc + k] = src[i];
@@ -405,7 +437,7 @@ void h_MultiVecLinearComb(void **x, void **y, int, int *start, int *end, double 
             for (int k = 0; k < nc; ++k) h[ncoef + k] = beta[(int64_t)k * incb];
         }
     }
-    double *d = c.small(ncoef + nc);
+    double *d = g.d;
     HIPCHECK(hipMemcpyAsync(d, h, sizeof(double) * (ncoef + (beta_mode == 2 ? nc : 0)),
                             hipMemcpyHostToDevice, c.stream));
     hipEvent_t ev = c.prof.begin(
@@ -416,7 +448,8 @@ void h_MultiVecLinearComb(void **x, void **y, int, int *start, int *end, double 
     c.prof.end(ev, c.stream);
     c.stats.lincomb_calls++;
     c.stats.lincomb_flops += 2.0 * (double)Y->nrows * nr * nc;
-    c.sync();  // the pinned staging buffer is reused by the next call
+    // no wait: the staging slot is reused only after this launch completes
+    c.staged(g);
 }
 
 // Inner products to a host array (app/app_lapack.c:299-321, 24-183)

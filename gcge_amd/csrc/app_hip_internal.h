@@ -61,6 +61,20 @@ struct Ctx {
     Profiler prof;
     // the device dense eigensolver's handle and buffers (dense_eig.cpp)
     void *dev_eig = nullptr;
+    // Upload staging for the coefficients of asynchronous slots: a ring of
+    // pinned host / device buffer pairs, each reused only after the event
+    // recorded behind the launches that read it has completed.
+    struct Staging {
+        double *h = nullptr, *d = nullptr;
+        int64_t cap = 0;
+        hipEvent_t ev = nullptr;
+        bool pending = false;
+    };
+    static constexpr int kStage = 8;
+    Staging stage[kStage];
+    int stage_next = 0;
+    Staging &staging(int64_t ndouble);
+    void staged(Staging &s);
     double *scratch(int64_t ndouble);
     double *small(int64_t ndouble);
     double *pinned(int64_t ndouble);

@@ -644,7 +644,9 @@ void dense_eigh(Solver &S, int n, double *M, double *w, double *Z)
 {
     Timer t(S, T_EIG);
     int backend = S.eig_backend;
-    if (backend < 0) backend = (n >= 96) ? GCGE_EIG_DEVICE : GCGE_EIG_HOST;
+    // the host solver wins below ~128 (no launches; measured 2.5 ms at 90,
+    // 10 ms at 150, against 5-10 ms for the device solver at 220-460)
+    if (backend < 0) backend = (n >= 128) ? GCGE_EIG_DEVICE : GCGE_EIG_HOST;
     int info;
     if (backend == GCGE_EIG_DEVICE)
         info = gcge::eigh_device(n, M, n, w, Z, n);

@@ -261,7 +261,7 @@ ThinPlan thin_plan(int64_t nrows, int m, int n)
     // reduced once per workgroup, so each should stream many rows (at most
     // 512 in total, so no straggler round)
     int64_t want = 512 / p.mtiles;
-    int64_t cap = (nrows + 8191) / 8192;
+    int64_t cap = (nrows + 1023) / 1024;  // at least 1024 rows per chunk
     int64_t nc = want < cap ? want : cap;
     if (nc < 1) nc = 1;
     p.rows_per_chunk = (nrows + nc - 1) / nc;

@@ -194,7 +194,9 @@ GramPlanM plan(int64_t nrows, int m, int n)
     // at most two workgroups per CU in total (256 CUs x 2 resident): one
     // more than that runs alone in a second round and doubles the time
     int64_t want = 512 / tiles;
-    int64_t cap = (nrows + 4095) / 4096;        // at least 4096 rows per chunk
+    // at least 512 rows (32 stages) per chunk: small problems still spread
+    // over many workgroups instead of a few walking all their rows
+    int64_t cap = (nrows + 511) / 512;
     int64_t nc = want < cap ? want : cap;
     if (nc < 1) nc = 1;
     int64_t rpc = (nrows + nc - 1) / nc;

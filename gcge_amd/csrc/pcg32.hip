@@ -24,8 +24,9 @@
 //   k_finish_pw    alpha = rho / p.w (0 for stopped columns)
 //   k_update_xr    d += alpha p, r -= alpha w, partial r.r per workgroup
 //   k_finish_rr    rho, stop flags, compacted tile list
-// and one 4-byte read of the active tile count by the host.  Reductions
-// are written per workgroup and summed in a fixed order (no atomics).
+// and a 4-byte read of the active tile count by the host every kCheck
+// iterations.  Reductions are written per workgroup and summed in a fixed
+// order (no atomics).
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -41,7 +42,8 @@ namespace gcge {
 namespace {
 
 constexpr int kWG = 256;
-constexpr int kC = 8;  // columns per interleaved tile
+constexpr int kC = 8;      // columns per interleaved tile
+constexpr int kCheck = 4;  // inner iterations between reads of the active count
 
 struct alignas(32) f8 {
     float v[kC];
@@ -536,9 +538,10 @@ void pcg32_solve(void *matA, void **mv_b, void **mv_x, int *start, int *end, OPS
                              (sigma != 0.f ? (double)Bs->nnz_true * 8.0 : 0.0);
     int it = 0;
     // algorithmic bytes and flops of the vector updates over the active
-    // tiles: p = r + beta p reads r (and p after the first iteration) and
-    // writes p; d += alpha p, r -= alpha w (with r.r) reads p, w, d, r and
-    // writes d, r
+    // tiles (as of the host's last read of the count, which can only
+    // overstate them: exact while no tile has stopped): p = r + beta p reads
+    // r (and p after the first iteration) and writes p; d += alpha p,
+    // r -= alpha w (with r.r) reads p, w, d, r and writes d, r
     const double tile_vec = 4.0 * kC * (double)n;
     while (it < S.max_iter && ntact > 0) {
         hipEvent_t evp = c.prof.begin(PROF_PCGVEC, (it == 0 ? 2.0 : 3.0) * tile_vec * ntact,
@@ -573,10 +576,16 @@ void pcg32_solve(void *matA, void **mv_b, void **mv_x, int *start, int *end, OPS
         k_finish_rr<<<1, 1024, 0, st>>>(nc, 1, (int)nb, S.partial, S.tiles, S.ntact, S.active,
                                         S.rho, S.rho_old, S.init_res, (float)S.rate,
                                         (float)S.tol);
-        HIPCHECK(hipMemcpyAsync(S.h_ntact, S.ntact, sizeof(int), hipMemcpyDeviceToHost, st));
-        c.sync();
-        ntact = *S.h_ntact;
         ++it;
+        // the host reads the active-tile count every kCheck iterations; in
+        // between it launches with the last count it read, an upper bound
+        // (tiles only ever stop), and the kernels skip the tiles beyond the
+        // device-side count
+        if (it % kCheck == 0 || it == S.max_iter) {
+            HIPCHECK(hipMemcpyAsync(S.h_ntact, S.ntact, sizeof(int), hipMemcpyDeviceToHost, st));
+            c.sync();
+            ntact = *S.h_ntact;
+        }
     }
     S.iters += it;
     // x += d in FP64

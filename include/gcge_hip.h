@@ -176,12 +176,18 @@ void HIPGCG_Free(OPS *ops);
 void HIPGCG_GetStats(OPS *ops, int64_t *stats, double *times);
 
 /* Dense symmetric eigensolver used by the Rayleigh-Ritz step:
- * GCGE_EIG_HOST (cyclic Jacobi on the host) or GCGE_EIG_DEVICE
+ * GCGE_EIG_HOST (gcge_host_dense_eigh below) or GCGE_EIG_DEVICE
  * (rocSOLVER dsyevd on the device).  Without a call, the device solver
- * is used for projected matrices of order 96 and above. */
+ * is used for projected matrices of order 128 and above. */
 #define GCGE_EIG_HOST 0
 #define GCGE_EIG_DEVICE 1
 void HIPGCG_SetDenseEigensolver(OPS *ops, int which);
+/* The host dense symmetric eigensolver (Householder tridiagonalization
+ * and implicit QR): all eigenvalues of the n x n matrix whose upper
+ * triangle A holds (lda), ascending in w, orthonormal eigenvectors in Z
+ * (ldz).  Returns 0, or 1 if the QR iteration did not converge.  Stands
+ * in for the reference's dsyevx call (src/ops_eig_sol_gcg.c:1207). */
+int gcge_host_dense_eigh(int n, const double *A, int lda, double *w, double *Z, int ldz);
 
 /* B-orthonormalization of the W block and of the initial block:
  * GCGE_ORTH_MGS (default) is the reference's block modified Gram-Schmidt

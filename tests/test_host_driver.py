@@ -133,3 +133,28 @@ def test_dense_orth(hip):
     proj = orig[:, 3:9] - Q0 @ (Q0.T @ orig[:, 3:9])
     coef = np.linalg.lstsq(Q[:, 3:], proj, rcond=None)[0]
     assert np.allclose(Q[:, 3:] @ coef, proj, atol=1e-10)
+
+
+@pytest.mark.parametrize("n", [1, 2, 5, 40, 130])
+def test_host_dense_eigh(hip, n):
+    """The host projected eigensolver against numpy, on graded spectra and
+    with repeated and clustered eigenvalues (the Ritz values of converged
+    pairs)."""
+    rng = np.random.default_rng(n)
+    Q, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    lam = np.sort(np.concatenate([np.full(n // 3, 0.25), 0.25 + 1e-13 * np.arange(n // 5),
+                                  rng.uniform(-3, 40, n - n // 3 - n // 5)]))
+    M = (Q * lam) @ Q.T
+    M = 0.5 * (M + M.T)
+    A = np.asfortranarray(np.triu(M) + np.tril(rng.standard_normal((n, n)), -1))  # upper read
+    w = np.zeros(n)
+    Z = np.zeros((n, n), order="F")
+    info = hip.gcge_host_dense_eigh(n, A.ctypes.data_as(abi.c_dbl_p), n,
+                                    w.ctypes.data_as(abi.c_dbl_p),
+                                    Z.ctypes.data_as(abi.c_dbl_p), n)
+    assert info == 0
+    scale = np.abs(lam).max()
+    assert np.all(np.diff(w) >= 0)
+    assert np.max(np.abs(w - np.linalg.eigvalsh(M))) <= 1e-13 * scale
+    assert np.max(np.abs(Z.T @ Z - np.eye(n))) <= 1e-13
+    assert np.max(np.abs(M @ Z - Z * w)) <= 1e-13 * scale
