@@ -163,6 +163,7 @@ EXPORTS = {
     "HIPGCG_MultiLinearSolverSetup_BlockPCG32": (None, [C.c_int, C.c_double, C.c_double,
                                                         C.c_void_p, C.c_double, OPS_p]),
     "HIPGCG_MultiLinearSolverFree_BlockPCG32": (None, [OPS_p]),
+    "HIPGCG_BlockPCG32_SetGraphs": (None, [OPS_p, C.c_int]),
     "gcge_request_footprint": (C.c_int64, [C.POINTER(Request)]),
     "gcge_service_create": (C.c_void_p, [C.c_int, C.c_int64]),
     "gcge_service_submit": (C.c_int64, [C.c_void_p, C.POINTER(Request), C.POINTER(Result)]),

@@ -438,11 +438,31 @@ struct PCG32 {
     int *tiles = nullptr, *ntact = nullptr, *active = nullptr;
     int *h_ntact = nullptr;
     int64_t iters = 0;
+    // kCheck inner iterations captured as one graph, for one key
+    bool use_graphs = true;
+    hipGraphExec_t gexec = nullptr;
+    struct GraphKey {
+        int ntact = -1, nc = 0;
+        const void *a = nullptr, *b = nullptr, *bv = nullptr;
+        float sigma = 0.f;
+        bool operator==(const GraphKey &o) const
+        {
+            return ntact == o.ntact && nc == o.nc && a == o.a && b == o.b && bv == o.bv &&
+                   sigma == o.sigma;
+        }
+    } gkey;
+    void drop_graph()
+    {
+        if (gexec) HIPCHECK(hipGraphExecDestroy(gexec));
+        gexec = nullptr;
+        gkey = GraphKey();
+    }
     void ensure(int64_t nrows, int nc)
     {
         const int nc8 = (nc + kC - 1) / kC * kC;
         if (nrows == n && nc8 <= ncap) return;
         release();
+        drop_graph();
         n = nrows;
         ncap = nc8;
         const size_t blkbytes = sizeof(float) * (size_t)n * ncap;
@@ -461,6 +481,7 @@ struct PCG32 {
     int64_t nparts_max() const { return (n + kWG - 1) / kWG; }
     void release()
     {
+        drop_graph();
         if (!x) return;
         ctx().sync();
         for (float *q : {x, r, p, w, rho, rho_old, init_res, alpha, partial}) HIPCHECK(hipFree(q));
@@ -543,18 +564,20 @@ void pcg32_solve(void *matA, void **mv_b, void **mv_x, int *start, int *end, OPS
     // r (and p after the first iteration) and writes p; d += alpha p,
     // r -= alpha w (with r.r) reads p, w, d, r and writes d, r
     const double tile_vec = 4.0 * kC * (double)n;
-    while (it < S.max_iter && ntact > 0) {
-        hipEvent_t evp = c.prof.begin(PROF_PCGVEC, (it == 0 ? 2.0 : 3.0) * tile_vec * ntact,
-                                      2.0 * kC * (double)n * ntact, st);
-        k_update_p<<<dim3(nb, ntact), kWG, 0, st>>>(n, S.tiles, S.ntact, S.rho, S.rho_old,
-                                                    it == 0 ? 1 : 0, S.r, S.p);
-        c.prof.end(evp, st);
-        hipEvent_t ev = c.prof.begin(PROF_SPMM32,
-                                     mat_bytes + 2.0 * 4.0 * kC * (double)ntact * n,
-                                     2.0 * (double)A->sell.nnz_true * kC * ntact, st);
-        // B's entries enter separately only for a shift by a B other than A
-        const bool two = sigma != 0.f && Bs->col != A->sell.col;
-        const float *vB = sigma != 0.f ? Bs->val32 : nullptr;
+    // B's entries enter separately only for a shift by a B other than A
+    const bool two = sigma != 0.f && Bs->col != A->sell.col;
+    const float *vB = sigma != 0.f ? Bs->val32 : nullptr;
+    // one inner iteration on st; `prof` records the kernel-class events
+    auto iteration = [&](int first, int nt, bool prof) {
+        hipEvent_t evp = prof ? c.prof.begin(PROF_PCGVEC, (first ? 2.0 : 3.0) * tile_vec * nt,
+                                             2.0 * kC * (double)n * nt, st)
+                              : nullptr;
+        k_update_p<<<dim3(nb, nt), kWG, 0, st>>>(n, S.tiles, S.ntact, S.rho, S.rho_old, first,
+                                                 S.r, S.p);
+        if (prof) c.prof.end(evp, st);
+        hipEvent_t ev = prof ? c.prof.begin(PROF_SPMM32, mat_bytes + 2.0 * 4.0 * kC * (double)nt * n,
+                                            2.0 * (double)A->sell.nnz_true * kC * nt, st)
+                             : nullptr;
 #define SPMM_PW(W, TWO)                                                                       \
     k_spmm_pw<W, TWO><<<nblk_row, kWG, 0, st>>>(n, A->sell.slice_ptr, A->sell.col,            \
                                                 A->sell.val32, Bs->col, vB, sigma, S.tiles,    \
@@ -565,22 +588,51 @@ void pcg32_solve(void *matA, void **mv_b, void **mv_x, int *start, int *end, OPS
             if (two) SPMM_PW(32, true); else SPMM_PW(32, false);
         }
 #undef SPMM_PW
-        c.prof.end(ev, st);
-        k_finish_pw<<<ntact * kC, 64, 0, st>>>(S.tiles, S.ntact, (int)nblk_row, S.partial, S.rho,
-                                               S.active, S.alpha);
-        hipEvent_t evx = c.prof.begin(PROF_PCGVEC, 6.0 * tile_vec * ntact,
-                                      6.0 * kC * (double)n * ntact, st);
-        k_update_xr<<<dim3(nb, ntact), kWG, 0, st>>>(n, S.tiles, S.ntact, S.alpha, S.p, S.w,
-                                                     S.x, S.r, S.partial);
-        c.prof.end(evx, st);
+        if (prof) c.prof.end(ev, st);
+        k_finish_pw<<<nt * kC, 64, 0, st>>>(S.tiles, S.ntact, (int)nblk_row, S.partial, S.rho,
+                                            S.active, S.alpha);
+        hipEvent_t evx = prof ? c.prof.begin(PROF_PCGVEC, 6.0 * tile_vec * nt,
+                                             6.0 * kC * (double)n * nt, st)
+                              : nullptr;
+        k_update_xr<<<dim3(nb, nt), kWG, 0, st>>>(n, S.tiles, S.ntact, S.alpha, S.p, S.w, S.x,
+                                                  S.r, S.partial);
+        if (prof) c.prof.end(evx, st);
         k_finish_rr<<<1, 1024, 0, st>>>(nc, 1, (int)nb, S.partial, S.tiles, S.ntact, S.active,
                                         S.rho, S.rho_old, S.init_res, (float)S.rate,
                                         (float)S.tol);
-        ++it;
+    };
+    while (it < S.max_iter && ntact > 0) {
         // the host reads the active-tile count every kCheck iterations; in
         // between it launches with the last count it read, an upper bound
         // (tiles only ever stop), and the kernels skip the tiles beyond the
-        // device-side count
+        // device-side count.  A whole group of kCheck iterations after the
+        // first is one graph launch (launch overhead dominates small
+        // problems), unless kernel timing is on.
+        const bool whole = it > 0 && it % kCheck == 0 && it + kCheck <= S.max_iter;
+        if (whole && S.use_graphs && !c.prof.on) {
+            PCG32::GraphKey key;
+            key.ntact = ntact;
+            key.nc = nc;
+            key.a = A->sell.col;
+            key.b = Bs->col;
+            key.bv = vB;
+            key.sigma = sigma;
+            if (!(S.gexec && key == S.gkey)) {
+                S.drop_graph();
+                hipGraph_t graph;
+                HIPCHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+                for (int k = 0; k < kCheck; ++k) iteration(0, ntact, false);
+                HIPCHECK(hipStreamEndCapture(st, &graph));
+                HIPCHECK(hipGraphInstantiate(&S.gexec, graph, nullptr, nullptr, 0));
+                HIPCHECK(hipGraphDestroy(graph));
+                S.gkey = key;
+            }
+            HIPCHECK(hipGraphLaunch(S.gexec, st));
+            it += kCheck;
+        } else {
+            iteration(it == 0 ? 1 : 0, ntact, true);
+            ++it;
+        }
         if (it % kCheck == 0 || it == S.max_iter) {
             HIPCHECK(hipMemcpyAsync(S.h_ntact, S.ntact, sizeof(int), hipMemcpyDeviceToHost, st));
             c.sync();
@@ -606,6 +658,16 @@ extern "C" void HIPGCG_MultiLinearSolverFree_BlockPCG32(OPS *ops)
     delete S;
     ops->multi_linear_solver_workspace = nullptr;
     ops->MultiLinearSolver = nullptr;
+}
+
+extern "C" void HIPGCG_BlockPCG32_SetGraphs(OPS *ops, int on)
+{
+    using namespace gcge;
+    if (ops->MultiLinearSolver != pcg32_solve || ops->multi_linear_solver_workspace == nullptr)
+        fail("HIPGCG_BlockPCG32_SetGraphs: no device BlockPCG installed", __FILE__, __LINE__);
+    PCG32 *S = static_cast<PCG32 *>(ops->multi_linear_solver_workspace);
+    S->use_graphs = on != 0;
+    if (!S->use_graphs) S->drop_graph();
 }
 
 extern "C" void HIPGCG_MultiLinearSolverSetup_BlockPCG32(int max_iter, double rate, double tol,

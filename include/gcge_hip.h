@@ -221,6 +221,10 @@ void HIPGCG_MultiLinearSolverSetup_BlockPCG32(int max_iter, double rate, double 
  * reference, whose BlockPCG workspace is a static (src/ops_lin_sol.c:
  * 439-465). */
 void HIPGCG_MultiLinearSolverFree_BlockPCG32(OPS *ops);
+/* Inner iterations between two reads of the active-column count run as one
+ * captured HIP graph (default on; off while kernel timing is enabled).  No
+ * counterpart in the reference; for callers that need plain stream launches. */
+void HIPGCG_BlockPCG32_SetGraphs(OPS *ops, int on);
 
 /* ------------------------------------------------------------------ */
 /* Solve service: concurrent requests on one GPU                       */

@@ -92,3 +92,32 @@ def test_shifted_operator(setup):
     # FP32 recursion: the true residual trails the recursive one by about
     # cond(A) * eps32 (cond ~ 1e3 here)
     assert rel.max() <= 1e-4, rel.max()
+
+
+@pytest.mark.parametrize("rate", [0.0, 0.2])
+def test_graph_replays_match_eager(setup, rate):
+    """Inner iterations replayed from a captured graph give the eager
+    launches' result bit for bit, call after call, with eager solves,
+    uploads and synchronizations in between (rate 0.2: tiles stop part way,
+    so the graph is captured again for the smaller active count)."""
+    table, lib, A, B, dA, dB = setup
+    n, nc = A.shape[0], 12
+    rng = np.random.default_rng(42)
+    b = rng.standard_normal((n, nc))
+    hb = table.create_multivec(nc, dA)
+    hx = table.create_multivec(nc, dA)
+    table.upload(hb, b)
+    lib.HIPGCG_MultiLinearSolverSetup_BlockPCG32(30, rate, 0.0, None, 0.0, table.ops)
+    outs = []
+    for graphs in (0, 1, 1, 0, 1, 1):
+        lib.HIPGCG_BlockPCG32_SetGraphs(table.ops, graphs)
+        table.upload(hx, np.zeros((n, nc)))
+        abi.slot(table.ops, "MultiLinearSolver")(dA.handle, hb, hx, ia(0, 0), ia(nc, nc),
+                                                 table.ops)
+        outs.append(table.download(hx, 0, nc, n))
+    for o in outs[1:]:
+        assert np.array_equal(o, outs[0])
+    r = np.linalg.norm(b - A @ outs[0], axis=0) / np.linalg.norm(b, axis=0)
+    assert r.max() < (1.0 if rate == 0.0 else rate * 1.01)
+    table.destroy_multivec(hb, nc)
+    table.destroy_multivec(hx, nc)
