@@ -111,6 +111,13 @@ class Result(C.Structure):
                 ("total_seconds", C.c_double), ("eval", c_dbl_p), ("evec", c_dbl_p)]
 
 
+class EpsOptions(C.Structure):
+    """gcge_eps_options (include/gcge_hip.h)."""
+    _fields_ = [("nev", C.c_int), ("ncv", C.c_int), ("tol", C.c_double), ("tol_type", C.c_int),
+                ("max_it", C.c_int), ("autoshift", C.c_int), ("mode", C.c_int),
+                ("print", C.c_int), ("device_random", C.c_int), ("seed", C.c_uint64)]
+
+
 # Exported functions of libgcge_hip.so and their signatures.
 EXPORTS = {
     "OPS_HIP_Set": (None, [OPS_p]),
@@ -171,6 +178,9 @@ EXPORTS = {
     "gcge_service_stats": (None, [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int),
                                   C.POINTER(C.c_int64)]),
     "gcge_service_destroy": (None, [C.c_void_p]),
+    "gcge_eps_parameters": (C.c_int, [C.c_int, C.c_int, c_int_p, c_int_p, c_int_p]),
+    "gcge_eps_solve": (C.c_int, [C.POINTER(CCSMat), C.POINTER(CCSMat), C.POINTER(EpsOptions),
+                                 c_dbl_p, c_dbl_p, c_int_p, c_int_p, c_int_p]),
 }
 
 GCGE_HIP_MAT_FP32 = 1
@@ -182,6 +192,13 @@ GCGE_ORTH_MGS = 0
 GCGE_ORTH_CHOLQR = 1
 GCGE_MODE_FP64 = 0
 GCGE_MODE_MIXED = 1
+GCGE_EPS_RELATIVE = 0
+GCGE_EPS_ABSOLUTE = 1
+GCGE_EPS_CONVERGED_TOL = 1
+GCGE_EPS_DIVERGED_ITS = -1
+GCGE_EPS_ERR_ARG = -10
+GCGE_EPS_ERR_NCV = -11
+GCGE_EPS_ERR_SIZE = -12
 
 _lib = None
 

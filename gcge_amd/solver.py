@@ -234,3 +234,49 @@ def solve_host_dense(A: np.ndarray, B: np.ndarray, nev: int, block_size: int, ne
     lib.HIPGCG_Free(ops)
     return SolveResult(ev, nconv.value, int(stats[0]), secs, int(stats[2]), int(stats[3]),
                        dict(zip(PHASES, list(times))), vecs, cq[0].value, cq[1].value)
+
+
+@dataclass
+class EpsResult:
+    eigr: np.ndarray
+    nconv: int
+    its: int
+    reason: int
+    evec: np.ndarray | None = None
+
+
+def eps_solve(A, B, nev: int, ncv: int = 0, tol: float = 0.0, absolute: bool = False,
+              max_it: int = 0, autoshift: int = -1, mode: str = "fp64",
+              device_random: bool = False, seed: int = 0,
+              want_vectors: bool = False) -> EpsResult:
+    """gcge_eps_solve: the SLEPc plug-in's policy and call sequence
+    (gcge.c:364-535) in one call."""
+    lib = abi.load()
+    keep = []
+
+    def ccs(M):
+        M = M.tocsc()
+        arrs = (np.ascontiguousarray(M.indptr, dtype=np.int32),
+                np.ascontiguousarray(M.indices, dtype=np.int32),
+                np.ascontiguousarray(M.data, dtype=np.float64))
+        keep.append(arrs)
+        return abi.CCSMat(arrs[2].ctypes.data_as(abi.c_dbl_p), arrs[1].ctypes.data_as(abi.c_int_p),
+                          arrs[0].ctypes.data_as(abi.c_int_p), M.shape[0], M.shape[1])
+    mA = ccs(A)
+    mB = ccs(B) if B is not None else None
+    opt = abi.EpsOptions(nev, ncv, tol, abi.GCGE_EPS_ABSOLUTE if absolute else abi.GCGE_EPS_RELATIVE,
+                         max_it, autoshift,
+                         abi.GCGE_MODE_MIXED if mode == "mixed" else abi.GCGE_MODE_FP64, 0,
+                         1 if device_random else 0, seed)
+    eigr = np.zeros(nev)
+    vec = np.zeros((A.shape[0], nev), order="F") if want_vectors else None
+    nconv, its, reason = C.c_int(), C.c_int(), C.c_int()
+    st = lib.gcge_eps_solve(C.byref(mA), C.byref(mB) if mB is not None else None, C.byref(opt),
+                            eigr.ctypes.data_as(abi.c_dbl_p),
+                            vec.ctypes.data_as(abi.c_dbl_p) if vec is not None else None,
+                            C.byref(nconv), C.byref(its), C.byref(reason))
+    if st != 0:
+        raise ValueError(f"gcge_eps_solve failed with {st}")
+    k = nconv.value
+    return EpsResult(eigr[:k].copy(), k, its.value, reason.value,
+                     vec[:, :k].copy() if vec is not None else None)

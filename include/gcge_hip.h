@@ -266,6 +266,43 @@ void gcge_service_stats(void *svc, int64_t *completed, int *max_concurrent,
 /* Finishes the queued requests and joins the workers. */
 void gcge_service_destroy(void *svc);
 
+/* ------------------------------------------------------------------ */
+/* EPS-style entry point                                               */
+/* ------------------------------------------------------------------ */
+/* One call with the parameter policy and call sequence of the reference's
+ * SLEPc plug-in (EPSSetUp_GCGE / EPSSolve_GCGE, gcge.c:364-535): block
+ * size, nevInit and nevMax from nev (and ncv), tolerances {1e-1 absolute,
+ * 1e-8 relative} with `tol` replacing one of them, gapMin 1e-5, autoshift
+ * on, srand(0) before the solve; nconv = min(converged, nev), reason and
+ * its as the plug-in reports them. */
+#define GCGE_EPS_RELATIVE 0
+#define GCGE_EPS_ABSOLUTE 1
+#define GCGE_EPS_CONVERGED_TOL 1   /* EPS_CONVERGED_TOL */
+#define GCGE_EPS_DIVERGED_ITS (-1) /* EPS_DIVERGED_ITS */
+#define GCGE_EPS_ERR_ARG (-10)
+#define GCGE_EPS_ERR_NCV (-11)     /* ncv inconsistent with nev (gcge.c:408-427) */
+#define GCGE_EPS_ERR_SIZE (-12)    /* nevMax above the matrix order (gcge.c:490-492) */
+#define GCGE_EPS_ERR_MATRIX (-13)  /* HIPCCS_Create rejected A or B */
+typedef struct {
+    int nev;
+    int ncv;           /* 0: the plug-in's default */
+    double tol;        /* 0: default */
+    int tol_type;      /* GCGE_EPS_RELATIVE (the plug-in's default test) or _ABSOLUTE */
+    int max_it;        /* 0: unlimited */
+    int autoshift;     /* -1: default (on); ignored (off) in mixed mode */
+    int mode;          /* GCGE_MODE_FP64 or GCGE_MODE_MIXED */
+    int print;         /* monitor lines through the table's Printf */
+    int device_random; /* 0: libc rand() after srand(0), as the plug-in */
+    uint64_t seed;     /* device generator seed when device_random */
+} gcge_eps_options;
+/* Block size, nevInit and nevMax the plug-in derives from nev and ncv;
+ * returns 0 or GCGE_EPS_ERR_NCV / _ARG.  No device use. */
+int gcge_eps_parameters(int nev, int ncv, int *block_size, int *nev_init, int *nev_max);
+/* Solves; eigr receives nconv values (array of nev), evec (or NULL) the
+ * N x nconv eigenvectors column-major.  Returns 0 or a GCGE_EPS_ERR_*. */
+int gcge_eps_solve(const gcge_ccs_mat *A, const gcge_ccs_mat *B, const gcge_eps_options *opt,
+                   double *eigr, double *evec, int *nconv, int *its, int *reason);
+
 #ifdef __cplusplus
 }
 #endif

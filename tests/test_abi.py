@@ -102,9 +102,11 @@ SERVICE_PROBE = r"""
 #include "gcge_hip.h"
 #define R(x) printf("req %s %zu\n", #x, offsetof(gcge_request, x));
 #define S(x) printf("res %s %zu\n", #x, offsetof(gcge_result, x));
+#define E(x) printf("eps %s %zu\n", #x, offsetof(gcge_eps_options, x));
 int main(void) {
   FIELDS
-  printf("req sizeof %zu\nres sizeof %zu\n", sizeof(gcge_request), sizeof(gcge_result));
+  printf("req sizeof %zu\nres sizeof %zu\neps sizeof %zu\n", sizeof(gcge_request),
+         sizeof(gcge_result), sizeof(gcge_eps_options));
   return 0;
 }
 """
@@ -112,7 +114,8 @@ int main(void) {
 
 def test_service_structs_match_ctypes(tmp_path):
     fields = " ".join([f"R({n})" for n, _ in abi.Request._fields_] +
-                      [f"S({n})" for n, _ in abi.Result._fields_])
+                      [f"S({n})" for n, _ in abi.Result._fields_] +
+                      [f"E({n})" for n, _ in abi.EpsOptions._fields_])
     src = tmp_path / "svc.c"
     src.write_text(SERVICE_PROBE.replace("FIELDS", fields))
     exe = tmp_path / "svc"
@@ -120,7 +123,7 @@ def test_service_structs_match_ctypes(tmp_path):
                    capture_output=True)
     out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split()
     got = {(out[i], out[i + 1]): int(out[i + 2]) for i in range(0, len(out), 3)}
-    for tag, cls in (("req", abi.Request), ("res", abi.Result)):
+    for tag, cls in (("req", abi.Request), ("res", abi.Result), ("eps", abi.EpsOptions)):
         for n, _ in cls._fields_:
             assert got[(tag, n)] == getattr(cls, n).offset, (tag, n)
         assert got[(tag, "sizeof")] == C.sizeof(cls)

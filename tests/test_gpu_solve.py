@@ -168,3 +168,27 @@ def test_27pt_32cubed_fp64_and_mixed_agree(hip):
     for r in (r64, rmx):
         rr = O.relative_residuals(A, B, r.eval[:r.nev_conv], r.evec[:, :r.nev_conv])
         assert rr.max() <= 1.01e-8
+
+
+def test_eps_entry_point_matches_reference_run(hip):
+    """gcge_eps_solve reproduces the plug-in's sequence: block policy,
+    tolerances, autoshift, srand(0) -- the reference's recorded run on the
+    same problem."""
+    prob = P.stencil_problem(32, 2, 10)
+    g = golden("2d_32", 1)
+    r = S.eps_solve(prob.A, prob.B, 10, want_vectors=True)
+    assert r.reason == abi.GCGE_EPS_CONVERGED_TOL and r.nconv == 10
+    assert abs(r.its - (g["num_iter"] + 1)) <= 3
+    ref = np.array(g["eval"][:10])
+    assert np.max(np.abs(r.eigr - ref) / ref) <= 1e-9
+    rr = O.relative_residuals(prob.A, prob.B, r.eigr, r.evec)
+    assert rr.max() <= 1.01e-8
+
+
+def test_eps_entry_point_mixed_and_diverged(hip):
+    prob = P.stencil_problem(32, 2, 10)
+    r = S.eps_solve(prob.A, prob.B, 10, mode="mixed")
+    exact = P.analytic_eigenvalues(32, 2, 10)
+    assert r.nconv == 10 and np.max(np.abs(r.eigr - exact) / exact) <= 1e-10
+    r1 = S.eps_solve(prob.A, prob.B, 10, max_it=1)
+    assert r1.reason == abi.GCGE_EPS_DIVERGED_ITS and r1.its == 1 and r1.nconv < 10

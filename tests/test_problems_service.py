@@ -92,3 +92,21 @@ def test_varcoef27_structure():
         assert np.isclose(-D[c, c + d], g * h, rtol=1e-14)
     bd = B.diagonal()
     assert B.nnz == 343 and bd.min() >= 0.5 and bd.max() <= 1.5
+
+
+def test_eps_parameter_policy():
+    """gcge_eps_parameters follows the plug-in's policy (gcge.c:394-431),
+    the one problems.slepc_block_policy restates, and its ncv checks."""
+    lib = abi.load()
+    out = [C.c_int() for _ in range(3)]
+    for nev in (1, 6, 7, 10, 19, 20, 100, 999, 1000, 3999, 4000):
+        assert lib.gcge_eps_parameters(nev, 0, *[C.byref(x) for x in out]) == 0
+        assert tuple(x.value for x in out) == P.slepc_block_policy(nev)
+    # ncv: nev <= 6 needs ncv == 2 nev; otherwise nevInit <= ncv <= nev + nevInit
+    assert lib.gcge_eps_parameters(4, 8, *[C.byref(x) for x in out]) == 0
+    assert lib.gcge_eps_parameters(4, 9, *[C.byref(x) for x in out]) == abi.GCGE_EPS_ERR_NCV
+    assert lib.gcge_eps_parameters(50, 45, *[C.byref(x) for x in out]) == 0
+    assert out[2].value == 45
+    assert lib.gcge_eps_parameters(50, 20, *[C.byref(x) for x in out]) == abi.GCGE_EPS_ERR_NCV
+    assert lib.gcge_eps_parameters(50, 81, *[C.byref(x) for x in out]) == abi.GCGE_EPS_ERR_NCV
+    assert lib.gcge_eps_parameters(0, 0, *[C.byref(x) for x in out]) == abi.GCGE_EPS_ERR_ARG
