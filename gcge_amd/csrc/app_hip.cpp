@@ -53,17 +53,28 @@ void hip_check(hipError_t e, const char *expr, const char *file, int line)
 // ----------------------------------------------------------------- context
 namespace {
 thread_local Ctx *t_ctx = nullptr;
+std::mutex g_ctx_mu;  // the registry of live contexts
+int g_live_ctx = 0;
 }
 
 Ctx &ctx()
 {
     if (t_ctx == nullptr) {
+        std::lock_guard<std::mutex> lk(g_ctx_mu);
         Ctx *x = new Ctx();
         HIPCHECK(hipGetDevice(&x->device));
         HIPCHECK(hipStreamCreateWithFlags(&x->stream, hipStreamNonBlocking));
         t_ctx = x;
+        ++g_live_ctx;
     }
     return *t_ctx;
+}
+
+std::unique_lock<std::mutex> sole_context_lock()
+{
+    std::unique_lock<std::mutex> lk(g_ctx_mu);
+    if (t_ctx == nullptr || g_live_ctx != 1) lk.unlock();
+    return lk;
 }
 
 void release_ctx()
@@ -85,6 +96,8 @@ void release_ctx()
     HIPCHECK(hipStreamDestroy(c->stream));
     delete c;
     t_ctx = nullptr;
+    std::lock_guard<std::mutex> lk(g_ctx_mu);
+    --g_live_ctx;
 }
 
 double *Ctx::scratch(int64_t ndouble)

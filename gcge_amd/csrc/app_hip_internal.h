@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <mutex>
 #include <vector>
 
 #include "kernels.h"
@@ -83,6 +84,14 @@ struct Ctx {
 Ctx &ctx();
 // Frees the calling thread's context (a later call creates a new one).
 void release_ctx();
+// Stream capture is process-wide state in the HIP runtime: while one thread
+// captures, allocations and synchronous calls made by other threads can be
+// refused or can invalidate the capture.  The returned lock owns the
+// context registry when the calling thread's context is the only one in
+// the process, and is empty otherwise; a caller captures only while it
+// owns the lock, and no other thread can create a context until it is
+// released.
+std::unique_lock<std::mutex> sole_context_lock();
 void release_dev_eig(void *p);
 
 // Device multivector: column-major nrows x ncols, leading dimension ldd.

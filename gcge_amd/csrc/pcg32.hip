@@ -607,17 +607,22 @@ void pcg32_solve(void *matA, void **mv_b, void **mv_x, int *start, int *end, OPS
         // (tiles only ever stop), and the kernels skip the tiles beyond the
         // device-side count.  A whole group of kCheck iterations after the
         // first is one graph launch (launch overhead dominates small
-        // problems), unless kernel timing is on.
+        // problems), unless kernel timing is on.  A graph is captured only
+        // while this thread's context is the process's only one; one
+        // captured earlier is replayed regardless.
         const bool whole = it > 0 && it % kCheck == 0 && it + kCheck <= S.max_iter;
-        if (whole && S.use_graphs && !c.prof.on) {
-            PCG32::GraphKey key;
-            key.ntact = ntact;
-            key.nc = nc;
-            key.a = A->sell.col;
-            key.b = Bs->col;
-            key.bv = vB;
-            key.sigma = sigma;
-            if (!(S.gexec && key == S.gkey)) {
+        PCG32::GraphKey key;
+        key.ntact = ntact;
+        key.nc = nc;
+        key.a = A->sell.col;
+        key.b = Bs->col;
+        key.bv = vB;
+        key.sigma = sigma;
+        const bool have = S.gexec && key == S.gkey;
+        std::unique_lock<std::mutex> solo;
+        if (whole && S.use_graphs && !c.prof.on && !have) solo = sole_context_lock();
+        if (whole && S.use_graphs && !c.prof.on && (have || solo.owns_lock())) {
+            if (!have) {
                 S.drop_graph();
                 hipGraph_t graph;
                 HIPCHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
@@ -626,6 +631,7 @@ void pcg32_solve(void *matA, void **mv_b, void **mv_x, int *start, int *end, OPS
                 HIPCHECK(hipGraphInstantiate(&S.gexec, graph, nullptr, nullptr, 0));
                 HIPCHECK(hipGraphDestroy(graph));
                 S.gkey = key;
+                solo.unlock();
             }
             HIPCHECK(hipGraphLaunch(S.gexec, st));
             it += kCheck;

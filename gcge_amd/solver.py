@@ -106,7 +106,8 @@ def solve(A, B, nev: int, block_size: int, nev_init: int, nev_max: int,
           tol=(1e-1, 1e-8), gap_min: float = 1e-5, shift: int = 1,
           max_iter: int = 1000, mode: str = "fp64", rand: str = "libc",
           seed: int = 0, eig: str = "auto", want_vectors: bool = False,
-          print_level: int = 0, mats=None, orth: str | None = None) -> SolveResult:
+          print_level: int = 0, mats=None, orth: str | None = None,
+          graphs: bool = True) -> SolveResult:
     """Smallest `nev` eigenpairs of A x = lambda B x on the GPU.
 
     mode "fp64": the reference-equivalent path, every operation in FP64.
@@ -117,6 +118,8 @@ def solve(A, B, nev: int, block_size: int, nev_init: int, nev_max: int,
     "device": counter-based generator on the device.
     orth "mgs" (the reference's orthogonalization) or "cholqr" (block
     Cholesky QR); default "mgs" for fp64 and "cholqr" for mixed.
+    graphs: replay the mixed path's BlockPCG iteration groups as captured
+    HIP graphs (False launches every kernel eagerly).
     `mats` may pass pre-uploaded (DeviceMatrix A, DeviceMatrix B).
     """
     lib = abi.load()
@@ -156,6 +159,8 @@ def solve(A, B, nev: int, block_size: int, nev_init: int, nev_max: int,
                              else abi.GCGE_ORTH_MGS)
     if mode == "mixed":
         lib.HIPGCG_MultiLinearSolverSetup_BlockPCG32(30, 1e-2, 1e-14, hB, 0.0, ops)
+        if not graphs:
+            lib.HIPGCG_BlockPCG32_SetGraphs(ops, 0)
     ev = np.zeros(nev_max)
     nconv = C.c_int(nev)
     lib.gcge_hip_synchronize()
