@@ -62,6 +62,21 @@ def make_problem(name: str):
     raise ValueError(name)
 
 
+def make_grid_problem(name: str):
+    """The grid form of make_problem's problem (assembled on the device)."""
+    from gcge_amd import problems as P
+    if name == "cfg2":
+        return P.config2_grid()
+    if name == "cfg1":
+        return P.config1_grid()
+    if name == "cfg3":
+        return P.config3_grid()
+    if name.startswith("3d_"):
+        n = int(name[3:])
+        return P.stencil_grid_problem(n, 3, 100, 120, 220, 220, f"3d_{n}")
+    return None
+
+
 def run_path(prob, n, dim, mode, steps, warmup, mats, want_check=True):
     from gcge_amd import abi, problems as P, solver as S
     lib = abi.load()
@@ -394,6 +409,24 @@ def main():
     mats = (S.DeviceMatrix(prob.A, fp32=True), S.DeviceMatrix(prob.B, fp32=True))
     t_upload = time.perf_counter() - t0
     log(f"upload {t_upload:.2f}s")
+    # the same matrices assembled on the device from the grid and the node
+    # coefficients; they must equal the uploaded ones bit for bit
+    assembly = None
+    gp = make_grid_problem(args.config)
+    if gp is not None:
+        abi.load().gcge_hip_synchronize()
+        t0 = time.perf_counter()
+        gA = S.device_operator(gp.A, fp32=True)
+        gB = S.device_operator(gp.B, fp32=True)
+        abi.load().gcge_hip_synchronize()
+        t_asm = time.perf_counter() - t0
+        same = gA.equal(mats[0]) and gB.equal(mats[1])
+        gA.free()
+        gB.free()
+        assembly = {"seconds": t_asm, "equal_to_upload": same}
+        log(f"device assembly {t_asm:.3f}s, equal to the upload: {same}")
+        if not same:
+            raise SystemExit("device-assembled matrices differ from the uploaded ones")
     if args.warmup == 0:
         # load the kernels and the dense eigensolver once (not a step)
         pp = P.stencil_problem(12, 3, 30)
@@ -452,6 +485,7 @@ def main():
                       "eval_first_last": r.get("eval_first_last")}
                   for m, r in results.items()},
         "upload_s": t_upload,
+        "device_assembly": assembly,
         "roofline": roof,
         "cpu_baseline": cpu,
     }

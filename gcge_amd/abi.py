@@ -118,6 +118,12 @@ class EpsOptions(C.Structure):
                 ("print", C.c_int), ("device_random", C.c_int), ("seed", C.c_uint64)]
 
 
+class GridOp(C.Structure):
+    """gcge_grid_op (include/gcge_hip.h)."""
+    _fields_ = [("kind", C.c_int), ("dim", C.c_int), ("points", C.c_int), ("n", C.c_int),
+                ("coef", c_dbl_p), ("delta_rel", C.c_double)]
+
+
 # Exported functions of libgcge_hip.so and their signatures.
 EXPORTS = {
     "OPS_HIP_Set": (None, [OPS_p]),
@@ -135,6 +141,8 @@ EXPORTS = {
                                                      C.c_void_p, OPS_p]),
     "HIPCCS_Create": (C.c_void_p, [C.POINTER(CCSMat), C.c_int]),
     "HIPCCS_Destroy": (None, [C.POINTER(C.c_void_p)]),
+    "HIPCCS_CreateGrid": (C.c_void_p, [C.POINTER(GridOp), C.c_int]),
+    "HIPCCS_Equal": (C.c_int, [C.c_void_p, C.c_void_p]),
     "HIPCCS_Info": (None, [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64),
                            C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
     "HIP_MultiVecSetFromHost": (None, [C.c_void_p, C.c_int, C.c_int, c_dbl_p, C.c_int64]),
@@ -184,6 +192,10 @@ EXPORTS = {
 }
 
 GCGE_HIP_MAT_FP32 = 1
+GCGE_GRID_LAPLACE = 0
+GCGE_GRID_MASS = 1
+GCGE_GRID_DIFFUSION = 2
+GCGE_GRID_DIAGONAL = 3
 GCGE_HIP_RAND_LIBC = 0
 GCGE_HIP_RAND_DEVICE = 1
 GCGE_EIG_HOST = 0

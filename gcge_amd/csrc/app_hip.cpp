@@ -70,6 +70,16 @@ Ctx &ctx()
     return *t_ctx;
 }
 
+uint64_t slice_hash(const std::vector<int64_t> &sp)
+{
+    uint64_t h = 1469598103934665603ull;  // FNV-1a over the slice pointers
+    for (int64_t v : sp) {
+        h ^= (uint64_t)v;
+        h *= 1099511628211ull;
+    }
+    return h;
+}
+
 std::unique_lock<std::mutex> sole_context_lock()
 {
     std::unique_lock<std::mutex> lk(g_ctx_mu);
@@ -706,12 +716,7 @@ void *HIPCCS_Create(const gcge_ccs_mat *host, int flags)
     }
     S.max_width = maxw;
     S.nnz = sp[S.nslices];
-    uint64_t h = 1469598103934665603ull;  // FNV-1a over the slice pointers
-    for (int64_t v : sp) {
-        h ^= (uint64_t)v;
-        h *= 1099511628211ull;
-    }
-    S.structure_hash = h;
+    S.structure_hash = slice_hash(sp);
     // Column r of a symmetric CCS matrix is row r: fill SELL by rows.
     std::vector<int> colv((size_t)(S.nnz > 0 ? S.nnz : 1));
     std::vector<double> valv((size_t)(S.nnz > 0 ? S.nnz : 1));

@@ -112,6 +112,9 @@ struct Mat {
     SellMat sell;
 };
 
+// Hash of a SELL matrix's slice pointers: equal hashes let A and B share
+// one gather pass.
+uint64_t slice_hash(const std::vector<int64_t> &sp);
 MV *mv_of(void **p, const char *where);
 Mat *mat_of(void *p, const char *where);
 MV *mv_create(int64_t nrows, int64_t ncols);

@@ -156,8 +156,17 @@ def varcoef_pair_2d(n: int, sigma: float, rng: np.random.Generator):
     kappa_i.  B = (8 I + Adj) / 12 as in the constant-coefficient pair, to
     which kappa = 1 reduces.  A is symmetric and strictly diagonally
     dominant, hence SPD."""
+    return varcoef_pair_2d_from(n, varcoef2d_coefficients(n, sigma, rng))
+
+
+def varcoef2d_coefficients(n: int, sigma: float, rng: np.random.Generator) -> np.ndarray:
+    """The node coefficients of varcoef_pair_2d: exp(sigma * z), z ~ N(0, 1)."""
+    return np.exp(sigma * rng.standard_normal(n * n))
+
+
+def varcoef_pair_2d_from(n: int, kappa: np.ndarray):
+    """varcoef_pair_2d for given node coefficients."""
     N = n * n
-    kappa = np.exp(sigma * rng.standard_normal(N))
     idx = np.arange(N).reshape(n, n)          # [y, x]
     rows, cols, vals = [], [], []
     diag = np.zeros(N)
@@ -235,9 +244,27 @@ def varcoef27_pair(n: int, sigma: float, rng: np.random.Generator,
 
     Built column by column in CCS directly: the 27 offsets in (dz, dy, dx)
     order are the columns of a row in ascending order."""
+    kappa, bdiag = varcoef27_coefficients(n, sigma, rng)
+    return varcoef27_from(n, kappa, delta_rel), diagonal_ccs(bdiag)
+
+
+def varcoef27_coefficients(n: int, sigma: float, rng: np.random.Generator):
+    """The node coefficients and the diagonal of B drawn by varcoef27_pair."""
     N = n ** 3
     kappa = np.exp(sigma * rng.standard_normal(N))
     bdiag = rng.uniform(0.5, 1.5, N)
+    return kappa, bdiag
+
+
+def diagonal_ccs(d: np.ndarray) -> sp.csc_matrix:
+    B = sp.diags(d).tocsc()
+    B.sort_indices()
+    return B
+
+
+def varcoef27_from(n: int, kappa: np.ndarray, delta_rel: float = 1e-8) -> sp.csc_matrix:
+    """The 27-point operator of varcoef27_pair for given node coefficients."""
+    N = n ** 3
     iz, iy, ix = (a.ravel() for a in np.indices((n, n, n), dtype=np.int32))
     offs = [(dz, dy, dx) for dz in (-1, 0, 1) for dy in (-1, 0, 1) for dx in (-1, 0, 1)]
 
@@ -280,10 +307,7 @@ def varcoef27_pair(n: int, sigma: float, rng: np.random.Generator,
     diag += delta_rel * diag.max()
     indices[pos_diag] = np.arange(N, dtype=np.int32)
     data[pos_diag] = diag
-    A = sp.csc_matrix((data, indices, indptr.astype(np.int32)), shape=(N, N))
-    B = sp.diags(bdiag).tocsc()
-    B.sort_indices()
-    return A, B
+    return sp.csc_matrix((data, indices, indptr.astype(np.int32)), shape=(N, N))
 
 
 def config3(n: int = 160, nev: int = 200, seed: int = SEED_BASE) -> CCSProblem:
@@ -294,3 +318,100 @@ def config3(n: int = 160, nev: int = 200, seed: int = SEED_BASE) -> CCSProblem:
     A, B = varcoef27_pair(n, 1.0, rng)
     bs, ni, nm = slepc_block_policy(nev)
     return CCSProblem(f"cfg3_27pt_{n}", A, B, nev, bs, ni, nm)
+
+
+# ---------------------------------------------------------------- grid forms
+# The same problems described by their grid and node coefficients, for
+# assembly on the device (HIPCCS_CreateGrid); to_ccs() gives the host CCS
+# the generators above build, which the device assembly reproduces bit for
+# bit.
+
+GRID_KINDS = {"laplace": 0, "mass": 1, "diffusion": 2, "diagonal": 3}
+
+
+@dataclass
+class GridOperator:
+    kind: str            # "laplace", "mass", "diffusion" or "diagonal"
+    dim: int
+    points: int          # 5 (2D), 7 or 27 (3D); ignored for "diagonal"
+    n: int               # interior points per side
+    coef: np.ndarray | None = None
+    delta_rel: float = 0.0
+
+    @property
+    def order(self) -> int:
+        return self.n ** self.dim
+
+    def to_ccs(self) -> sp.csc_matrix:
+        if self.kind in ("laplace", "mass"):
+            return stencil_pair(self.n, self.dim)[0 if self.kind == "laplace" else 1]
+        if self.kind == "diffusion" and self.dim == 2:
+            return varcoef_pair_2d_from(self.n, self.coef)[0]
+        if self.kind == "diffusion" and self.dim == 3 and self.points == 27:
+            return varcoef27_from(self.n, self.coef, self.delta_rel)
+        if self.kind == "diagonal":
+            return diagonal_ccs(self.coef)
+        raise ValueError(f"no host form for {self}")
+
+
+@dataclass
+class GridProblem:
+    name: str
+    A: GridOperator
+    B: GridOperator | None
+    nev: int
+    block_size: int
+    nev_init: int
+    nev_max: int
+    tol_abs: float = 1e-1
+    tol_rel: float = 1e-8
+    gap_min: float = 1e-5
+
+    @property
+    def n(self) -> int:
+        return self.A.order
+
+    def to_ccs(self) -> CCSProblem:
+        return CCSProblem(self.name, self.A.to_ccs(),
+                          self.B.to_ccs() if self.B is not None else None, self.nev,
+                          self.block_size, self.nev_init, self.nev_max, self.tol_abs,
+                          self.tol_rel, self.gap_min)
+
+
+def stencil_grid_problem(n: int, dim: int, nev: int, block_size: int, nev_init: int,
+                         nev_max: int, name: str) -> GridProblem:
+    pts = 5 if dim == 2 else 7
+    return GridProblem(name, GridOperator("laplace", dim, pts, n),
+                       GridOperator("mass", dim, pts, n), nev, block_size, nev_init, nev_max)
+
+
+def config1_grid() -> GridProblem:
+    """config1() as a grid problem."""
+    bs, ni, nm = slepc_block_policy(20)
+    return stencil_grid_problem(128, 2, 20, bs, ni, nm, "cfg1_2d_128")
+
+
+def config2_grid() -> GridProblem:
+    """config2() as a grid problem."""
+    return stencil_grid_problem(128, 3, 100, 120, 220, 220, "cfg2_3d_128")
+
+
+def config3_grid(n: int = 160, nev: int = 200, seed: int = SEED_BASE) -> GridProblem:
+    """config3() as a grid problem (the same random draws)."""
+    rng = np.random.default_rng([seed, 3])
+    kappa, bdiag = varcoef27_coefficients(n, 1.0, rng)
+    bs, ni, nm = slepc_block_policy(nev)
+    return GridProblem(f"cfg3_27pt_{n}", GridOperator("diffusion", 3, 27, n, kappa, 1e-8),
+                       GridOperator("diagonal", 3, 0, n, bdiag), nev, bs, ni, nm)
+
+
+def config4_grid(i: int, seed: int = SEED_BASE) -> GridProblem:
+    """config4_problem(i) as a grid problem (the same random draws)."""
+    rng = np.random.default_rng([seed, i])
+    N = int(rng.integers(2048, 65536 + 1))
+    n = int(round(math.sqrt(N)))
+    nev = int(rng.integers(8, 64 + 1))
+    kappa = varcoef2d_coefficients(n, 0.5, rng)
+    bs, ni, nm = slepc_block_policy(nev)
+    return GridProblem(f"cfg4_{i}_2d{n}", GridOperator("diffusion", 2, 5, n, kappa),
+                       GridOperator("mass", 2, 5, n), nev, bs, ni, nm)

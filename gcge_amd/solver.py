@@ -15,7 +15,7 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-from . import abi
+from . import abi, problems
 
 PHASES = ("initX", "checkconv", "compP", "compRR", "rr_matW", "dsyevx", "compRV",
           "compW", "compX", "linsol")
@@ -52,6 +52,30 @@ class DeviceMatrix:
         if not h:
             raise ValueError("HIPCCS_Create rejected the matrix")
         self.handle = C.c_void_p(h)
+
+    @classmethod
+    def grid(cls, kind: int, dim: int, points: int, n: int, coef=None,
+             delta_rel: float = 0.0, fp32: bool = False) -> "DeviceMatrix":
+        """A grid operator assembled on the device (HIPCCS_CreateGrid):
+        the same handle HIPCCS_Create gives for the host CCS of the
+        operator, from the grid size and the node coefficients only."""
+        lib = abi.load()
+        c = None
+        if coef is not None:
+            c = np.ascontiguousarray(coef, dtype=np.float64)
+        op = abi.GridOp(kind, dim, points, n,
+                        c.ctypes.data_as(abi.c_dbl_p) if c is not None else None, delta_rel)
+        h = lib.HIPCCS_CreateGrid(C.byref(op), abi.GCGE_HIP_MAT_FP32 if fp32 else 0)
+        if not h:
+            raise ValueError("HIPCCS_CreateGrid rejected the operator")
+        self = cls.__new__(cls)
+        self.n = n ** dim
+        self.handle = C.c_void_p(h)
+        return self
+
+    def equal(self, other: "DeviceMatrix") -> bool:
+        """True when both handles hold the same device arrays."""
+        return bool(abi.load().HIPCCS_Equal(self.handle, other.handle))
 
     def info(self):
         lib = abi.load()
@@ -183,9 +207,23 @@ def solve(A, B, nev: int, block_size: int, nev_init: int, nev_max: int,
                        dict(zip(PHASES, list(times))), vecs, cq[0].value, cq[1].value)
 
 
+def device_operator(op, fp32: bool = False) -> DeviceMatrix:
+    """A problems.GridOperator assembled on the device."""
+    return DeviceMatrix.grid(problems.GRID_KINDS[op.kind], op.dim, op.points, op.n, op.coef,
+                             op.delta_rel, fp32)
+
+
 def solve_problem(prob, **kw) -> SolveResult:
-    """solve() with the sizes and tolerances of a problems.CCSProblem."""
-    return solve(prob.A, prob.B, prob.nev, prob.block_size, prob.nev_init, prob.nev_max,
+    """solve() with the sizes and tolerances of a problems.CCSProblem, or of
+    a problems.GridProblem, whose matrices are then assembled on the
+    device."""
+    if isinstance(prob, problems.GridProblem) and kw.get("mats") is None:
+        fp32 = kw.get("mode", "fp64") == "mixed"
+        kw["mats"] = (device_operator(prob.A, fp32),
+                      device_operator(prob.B, fp32) if prob.B is not None else None)
+    A = getattr(prob, "A", None) if not isinstance(prob, problems.GridProblem) else None
+    B = getattr(prob, "B", None) if not isinstance(prob, problems.GridProblem) else None
+    return solve(A, B, prob.nev, prob.block_size, prob.nev_init, prob.nev_max,
                  tol=(prob.tol_abs, prob.tol_rel), gap_min=prob.gap_min, **kw)
 
 

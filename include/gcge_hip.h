@@ -79,6 +79,46 @@ void HIPCCS_Destroy(void **mat);
 void HIPCCS_Info(void *mat, int64_t *nrows, int64_t *nnz, int64_t *padded,
                  int64_t *bytes);
 
+/* Grid operators assembled on the device, without a host CCS.  The
+ * reference builds its matrices on the host and hands over a CCSMAT
+ * (app/app_ccs.h:20-24); these give the handle HIPCCS_Create would return
+ * for the same operator, bit for bit (HIPCCS_Equal), from the grid size
+ * and, for variable coefficients, the N node values only.
+ *
+ * Interior grid of n points per side, N = n^dim, Dirichlet boundary,
+ * point index x + n y + n^2 z, rows in ascending column order.
+ *   LAPLACE   (dim 2, 5 points; dim 3, 7 points): 2 dim I - Adj.
+ *   MASS      (same stencils): (4 dim I + Adj) / (6 dim).
+ *   DIFFUSION dim 2, 5 points: edge weight w_ij = 2 k_i k_j / (k_i + k_j)
+ *             from the node coefficients k = coef; A_ij = -w_ij, A_ii = the
+ *             weights at i (in the order +x, -x, +y, -y) plus k_i per
+ *             missing neighbour.
+ *             dim 3, 27 points: the neighbour at offset d couples with
+ *             g(d) 2 k_i k_j / (k_i + k_j), g(d) = 1 / |d|^2; A_ii sums, in
+ *             ascending offset order, the weights and g(d) k_i for each
+ *             missing neighbour, then adds delta_rel times the largest
+ *             such diagonal.
+ *   DIAGONAL  (any dim, points ignored): diag(coef).
+ * coef is a host array of N doubles (DIFFUSION, DIAGONAL), else unused.
+ * A single-precision copy is kept when flags & GCGE_HIP_MAT_FP32.
+ * Returns NULL on an unsupported combination or N >= 2^31. */
+#define GCGE_GRID_LAPLACE 0
+#define GCGE_GRID_MASS 1
+#define GCGE_GRID_DIFFUSION 2
+#define GCGE_GRID_DIAGONAL 3
+typedef struct gcge_grid_op {
+    int kind;
+    int dim;
+    int points;
+    int n;
+    const double *coef;
+    double delta_rel;
+} gcge_grid_op;
+void *HIPCCS_CreateGrid(const gcge_grid_op *op, int flags);
+/* 1 when two matrix handles hold the same device arrays (slice layout,
+ * columns, values and the FP32 copy if both have one), 0 otherwise. */
+int HIPCCS_Equal(void *a, void *b);
+
 /* Host <-> device copies of columns [start, end) of a multivector; the
  * host array is column-major with leading dimension ld. */
 void HIP_MultiVecSetFromHost(void **mv, int start, int end, const double *src,

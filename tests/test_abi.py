@@ -103,10 +103,12 @@ SERVICE_PROBE = r"""
 #define R(x) printf("req %s %zu\n", #x, offsetof(gcge_request, x));
 #define S(x) printf("res %s %zu\n", #x, offsetof(gcge_result, x));
 #define E(x) printf("eps %s %zu\n", #x, offsetof(gcge_eps_options, x));
+#define G(x) printf("grid %s %zu\n", #x, offsetof(gcge_grid_op, x));
 int main(void) {
   FIELDS
-  printf("req sizeof %zu\nres sizeof %zu\neps sizeof %zu\n", sizeof(gcge_request),
-         sizeof(gcge_result), sizeof(gcge_eps_options));
+  printf("req sizeof %zu\nres sizeof %zu\neps sizeof %zu\ngrid sizeof %zu\n",
+         sizeof(gcge_request), sizeof(gcge_result), sizeof(gcge_eps_options),
+         sizeof(gcge_grid_op));
   return 0;
 }
 """
@@ -115,7 +117,8 @@ int main(void) {
 def test_service_structs_match_ctypes(tmp_path):
     fields = " ".join([f"R({n})" for n, _ in abi.Request._fields_] +
                       [f"S({n})" for n, _ in abi.Result._fields_] +
-                      [f"E({n})" for n, _ in abi.EpsOptions._fields_])
+                      [f"E({n})" for n, _ in abi.EpsOptions._fields_] +
+                      [f"G({n})" for n, _ in abi.GridOp._fields_])
     src = tmp_path / "svc.c"
     src.write_text(SERVICE_PROBE.replace("FIELDS", fields))
     exe = tmp_path / "svc"
@@ -123,7 +126,8 @@ def test_service_structs_match_ctypes(tmp_path):
                    capture_output=True)
     out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split()
     got = {(out[i], out[i + 1]): int(out[i + 2]) for i in range(0, len(out), 3)}
-    for tag, cls in (("req", abi.Request), ("res", abi.Result), ("eps", abi.EpsOptions)):
+    for tag, cls in (("req", abi.Request), ("res", abi.Result), ("eps", abi.EpsOptions),
+                     ("grid", abi.GridOp)):
         for n, _ in cls._fields_:
             assert got[(tag, n)] == getattr(cls, n).offset, (tag, n)
         assert got[(tag, "sizeof")] == C.sizeof(cls)

@@ -110,3 +110,47 @@ def test_eps_parameter_policy():
     assert lib.gcge_eps_parameters(50, 20, *[C.byref(x) for x in out]) == abi.GCGE_EPS_ERR_NCV
     assert lib.gcge_eps_parameters(50, 81, *[C.byref(x) for x in out]) == abi.GCGE_EPS_ERR_NCV
     assert lib.gcge_eps_parameters(0, 0, *[C.byref(x) for x in out]) == abi.GCGE_EPS_ERR_ARG
+
+
+def _same_ccs(X, Y):
+    X, Y = X.tocsc(), Y.tocsc()
+    return (np.array_equal(X.indptr, Y.indptr) and np.array_equal(X.indices, Y.indices)
+            and np.array_equal(X.data.view(np.int64), Y.data.view(np.int64)))
+
+
+def test_grid_forms_reproduce_the_generators():
+    """The grid descriptions draw the same coefficients and, through
+    to_ccs(), give the generators' matrices bit for bit (the device
+    assembly is checked against these)."""
+    for i in (0, 9):
+        a, g = P.config4_problem(i), P.config4_grid(i)
+        c = g.to_ccs()
+        assert _same_ccs(a.A, c.A) and _same_ccs(a.B, c.B)
+        assert (a.nev, a.block_size, a.nev_init, a.nev_max) == \
+            (g.nev, g.block_size, g.nev_init, g.nev_max)
+    a, g = P.config3(10), P.config3_grid(10)
+    c = g.to_ccs()
+    assert _same_ccs(a.A, c.A) and _same_ccs(a.B, c.B) and g.n == 1000
+    a = P.stencil_problem(9, 3, 100, 120, 220, 220)
+    c = P.stencil_grid_problem(9, 3, 100, 120, 220, 220, "s").to_ccs()
+    assert _same_ccs(a.A, c.A) and _same_ccs(a.B, c.B)
+
+
+def test_create_grid_rejects_unsupported_operators():
+    """Rejected before any device call: unsupported stencils, missing
+    coefficients, empty grids and orders beyond int32 column indices."""
+    lib = abi.load()
+    kappa = np.ones(27)
+    p = kappa.ctypes.data_as(abi.c_dbl_p)
+    bad = [abi.GridOp(abi.GCGE_GRID_LAPLACE, 3, 27, 3, None, 0.0),
+           abi.GridOp(abi.GCGE_GRID_MASS, 2, 7, 3, None, 0.0),
+           abi.GridOp(abi.GCGE_GRID_DIFFUSION, 3, 7, 3, p, 0.0),
+           abi.GridOp(abi.GCGE_GRID_DIFFUSION, 3, 27, 3, None, 0.0),
+           abi.GridOp(abi.GCGE_GRID_DIAGONAL, 3, 0, 3, None, 0.0),
+           abi.GridOp(abi.GCGE_GRID_LAPLACE, 2, 5, 0, None, 0.0),
+           abi.GridOp(abi.GCGE_GRID_LAPLACE, 4, 5, 3, None, 0.0),
+           abi.GridOp(7, 2, 5, 3, None, 0.0),
+           abi.GridOp(abi.GCGE_GRID_LAPLACE, 3, 7, 1291, None, 0.0)]   # 1291^3 > 2^31
+    for op in bad:
+        assert not lib.HIPCCS_CreateGrid(C.byref(op), 0), (op.kind, op.dim, op.points, op.n)
+    assert not lib.HIPCCS_CreateGrid(None, 0)
