@@ -13,6 +13,7 @@
 #include <cstdint>
 
 #include "kernels.h"
+#include "sell_rows.h"
 
 namespace gcge {
 namespace {
@@ -29,6 +30,8 @@ inline unsigned grid1d(int64_t n, int per_thread = 1)
 // SpMM, SELL-64 row gather.  One thread per row keeps the row's entries
 // in registers (W = compile-time bound on the slice width) and sweeps all
 // columns of the block in tiles of T, so the matrix is read once per call.
+// The tile loop runs in a body compiled for the slice's exact width
+// (for_width), so all w x T gathers of a tile are in flight together.
 // ---------------------------------------------------------------------
 template <typename V, int W, int T>
 __global__ __launch_bounds__(kWG) void k_spmm_sell(
@@ -36,50 +39,54 @@ __global__ __launch_bounds__(kWG) void k_spmm_sell(
     const V *__restrict__ val, const V *__restrict__ X, int64_t ldx, V *__restrict__ Y,
     int64_t ldy, int ncols)
 {
+    static_assert(kWG == 256, "one wavefront per slice, four per workgroup");
     // plain block order (remapping blocks to give each XCD a contiguous row
     // range measured ~10 % slower for this gather)
-    const int64_t blk = blockIdx.x;
-    const int64_t row = blk * kWG + threadIdx.x;
+    const int64_t row = (int64_t)blockIdx.x * kWG + threadIdx.x;
     if (row >= nrows) return;
-    const int64_t s = row >> 6;
-    const int lane = (int)(row & 63);
+    const int64_t s = wave_slice();
+    const int lane = (int)(threadIdx.x & 63);
     const int64_t base = slice_ptr[s];
-    const int w = (int)((slice_ptr[s + 1] - base) >> 6);
-    int cj[W];
+    const int w = __builtin_amdgcn_readfirstlane((int)((slice_ptr[s + 1] - base) >> 6));
+    // gathers address a wave-uniform column base plus a 32-bit byte offset
+    // per entry (one VGPR each instead of a 64-bit address per load)
+    uint32_t oj[W];
     V vj[W];
 #pragma unroll
     for (int j = 0; j < W; ++j) {
-        if (j < w) {
-            cj[j] = col[base + (int64_t)j * 64 + lane];
+        if (j < w) {  // wave-uniform
+            oj[j] = (uint32_t)col[base + (int64_t)j * 64 + lane] * (uint32_t)sizeof(V);
             vj[j] = val[base + (int64_t)j * 64 + lane];
-        } else {
-            cj[j] = (int)row;
-            vj[j] = V(0);
         }
     }
-    int c0 = 0;
-    for (; c0 + T <= ncols; c0 += T) {
-        V acc[T];
+    // one buffer resource per column: a column holds nrows values
+    const uint32_t colbytes = (uint32_t)(nrows * (int64_t)sizeof(V));
+    for_width<W>(w, [&](auto kw) {
+        constexpr int K = decltype(kw)::value;
+        int c0 = 0;
+        for (; c0 + T <= ncols; c0 += T) {
+            __amdgpu_buffer_rsrc_t rs[T];
 #pragma unroll
-        for (int t = 0; t < T; ++t) acc[t] = V(0);
+            for (int t = 0; t < T; ++t) rs[t] = buffer_rsrc(X + (int64_t)(c0 + t) * ldx, colbytes);
+            V acc[T];
 #pragma unroll
-        for (int j = 0; j < W; ++j) {
-            if (j < w) {
-                const V *xp = X + cj[j] + (int64_t)c0 * ldx;
+            for (int t = 0; t < T; ++t) acc[t] = V(0);
 #pragma unroll
-                for (int t = 0; t < T; ++t) acc[t] += vj[j] * xp[(int64_t)t * ldx];
+            for (int j = 0; j < K; ++j) {
+#pragma unroll
+                for (int t = 0; t < T; ++t) acc[t] += vj[j] * buffer_load<V>(rs[t], oj[j]);
             }
+#pragma unroll
+            for (int t = 0; t < T; ++t) Y[row + (int64_t)(c0 + t) * ldy] = acc[t];
         }
+        for (; c0 < ncols; ++c0) {
+            const __amdgpu_buffer_rsrc_t r1 = buffer_rsrc(X + (int64_t)c0 * ldx, colbytes);
+            V acc = V(0);
 #pragma unroll
-        for (int t = 0; t < T; ++t) Y[row + (int64_t)(c0 + t) * ldy] = acc[t];
-    }
-    for (; c0 < ncols; ++c0) {
-        V acc = V(0);
-#pragma unroll
-        for (int j = 0; j < W; ++j)
-            if (j < w) acc += vj[j] * X[cj[j] + (int64_t)c0 * ldx];
-        Y[row + (int64_t)c0 * ldy] = acc;
-    }
+            for (int j = 0; j < K; ++j) acc += vj[j] * buffer_load<V>(r1, oj[j]);
+            Y[row + (int64_t)c0 * ldy] = acc;
+        }
+    });
 }
 
 // Rows of any width: entries re-read from memory for each column.
@@ -118,7 +125,12 @@ void launch_spmm(const SellMat &A, const V *val, const V *X, int64_t ldx, V *Y,
 {
     if (ncols <= 0 || A.nrows == 0) return;
     const unsigned g = grid1d(A.nrows);
-    if (A.max_width <= 8)
+    // the register kernels gather by 32-bit byte offsets within a column
+    const bool small = A.nrows * (int64_t)sizeof(V) <= (int64_t)UINT32_MAX;
+    if (!small)
+        k_spmm_sell_wide<V><<<g, kWG, 0, s>>>(A.nrows, A.slice_ptr, A.col, val, X, ldx, Y, ldy,
+                                              ncols);
+    else if (A.max_width <= 8)
         k_spmm_sell<V, 8, 4><<<g, kWG, 0, s>>>(A.nrows, A.slice_ptr, A.col, val, X, ldx, Y,
                                                ldy, ncols);
     else if (A.max_width <= 16)

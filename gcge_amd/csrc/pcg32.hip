@@ -37,6 +37,7 @@
 #include "app_hip_internal.h"
 #include "gcge_hip.h"
 #include "kernels.h"
+#include "sell_rows.h"
 
 namespace gcge {
 namespace {
@@ -85,6 +86,24 @@ __device__ __forceinline__ void store8(float *p, const f8 &x)
     *reinterpret_cast<f8 *>(p) = x;
 }
 
+typedef float f4v __attribute__((ext_vector_type(4)));
+
+// load8 through a buffer resource at a byte offset (two 16-byte loads).
+// The loaded vectors are bit-cast whole: a bit cast of one element of the
+// integer vector reads the vector's first element for every index.
+__device__ __forceinline__ f8 buffer_load8(__amdgpu_buffer_rsrc_t r, uint32_t off)
+{
+    const f4v a = __builtin_bit_cast(f4v, __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0));
+    const f4v b = __builtin_bit_cast(f4v, __builtin_amdgcn_raw_buffer_load_b128(r, off + 16, 0, 0));
+    f8 x;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        x.v[i] = a[i];
+        x.v[4 + i] = b[i];
+    }
+    return x;
+}
+
 // Residual of the FP64 system, rounded to FP32, for every column c < nc:
 // R[c] = fl32(b[:, c] - (A + sigma B) x[:, c]) with the products in FP64
 // from the FP64 matrix values; partial[c * nblk + block] = sum of R^2.
@@ -100,35 +119,37 @@ __global__ __launch_bounds__(kWG) void k_resid64(
     int64_t ldb, const double *__restrict__ Xv, int64_t ldx, float *__restrict__ R,
     float *__restrict__ partial)
 {
+    static_assert(kWG == 256, "one wavefront per slice, four per workgroup");
     __shared__ float red[(kWG / 64 + 1) * kC];
     const int64_t nblk = gridDim.x;
     // plain block order (remapping blocks to give each XCD a contiguous row
     // range measured ~10 % slower for this gather)
-    const int64_t blk = blockIdx.x;
-    const int64_t row = blk * kWG + threadIdx.x;
+    const int64_t row = (int64_t)blockIdx.x * kWG + threadIdx.x;
     const bool live = row < n;
-    int ca[W], cb[W];
-    double va[W], vb[W];
+    // the wavefront's slice width; every lane of a slice has stored entries
+    // (padding for rows past n), so all lanes gather and only live ones
+    // store
+    const int64_t s = wave_slice();
+    const int lane = (int)(threadIdx.x & 63);
     int w = 0;
-    if (live) {
-        const int64_t s = row >> 6;
-        const int lane = (int)(row & 63);
-        const int64_t base = slice_ptr[s];
-        w = (int)((slice_ptr[s + 1] - base) >> 6);
+    int64_t base = 0;
+    if (s < (n + 63) / 64) {  // wave-uniform
+        base = slice_ptr[s];
+        w = __builtin_amdgcn_readfirstlane((int)((slice_ptr[s + 1] - base) >> 6));
+    }
+    uint32_t oa[W], ob[W];
+    double va[W], vb[W];
 #pragma unroll
-        for (int j = 0; j < W; ++j) {
-            if (j < w) {
-                const int64_t e = base + (int64_t)j * 64 + lane;
-                ca[j] = colA[e];
-                va[j] = valA[e];
-                cb[j] = SHIFT ? colB[e] : ca[j];
-                vb[j] = SHIFT ? sigma * valB[e] : 0.0;
-            } else {
-                ca[j] = cb[j] = (int)row;
-                va[j] = vb[j] = 0.0;
-            }
+    for (int j = 0; j < W; ++j) {
+        if (j < w) {  // wave-uniform
+            const int64_t e = base + (int64_t)j * 64 + lane;
+            oa[j] = (uint32_t)colA[e] * 8u;
+            va[j] = valA[e];
+            ob[j] = SHIFT ? (uint32_t)colB[e] * 8u : oa[j];
+            vb[j] = SHIFT ? sigma * valB[e] : 0.0;
         }
     }
+    const uint32_t colbytes = (uint32_t)(n * 8);
     const int ntile = (nc + kC - 1) / kC;
     for (int tile = 0; tile < ntile; ++tile) {
         float sq[kC];
@@ -138,26 +159,28 @@ __global__ __launch_bounds__(kWG) void k_resid64(
             sq[t] = 0.f;
             r8.v[t] = 0.f;
         }
-        if (live) {
+        for_width<W>(w, [&](auto kw) {
+            constexpr int K = decltype(kw)::value;
 #pragma unroll
             for (int t = 0; t < kC; ++t) {
                 const int c = tile * kC + t;
-                if (c < nc) {
-                    const double *xc = Xv + (int64_t)c * ldx;
+                if (c < nc) {  // wave-uniform
+                    const auto xr = buffer_rsrc(Xv + (int64_t)c * ldx, colbytes);
                     double acc = 0.0;
 #pragma unroll
-                    for (int j = 0; j < W; ++j)
-                        if (j < w) {
-                            acc += va[j] * xc[ca[j]];
-                            if (SHIFT) acc += vb[j] * xc[cb[j]];
-                        }
-                    const float r = (float)(Bv[row + (int64_t)c * ldb] - acc);
-                    r8.v[t] = r;
-                    sq[t] = r * r;
+                    for (int j = 0; j < K; ++j) {
+                        acc += va[j] * buffer_load<double>(xr, oa[j]);
+                        if (SHIFT) acc += vb[j] * buffer_load<double>(xr, ob[j]);
+                    }
+                    if (live) {
+                        const float r = (float)(Bv[row + (int64_t)c * ldb] - acc);
+                        r8.v[t] = r;
+                        sq[t] = r * r;
+                    }
                 }
             }
-            store8(R + ((int64_t)tile * n + row) * kC, r8);
-        }
+        });
+        if (live) store8(R + ((int64_t)tile * n + row) * kC, r8);
         block_sum8(sq, red);
         if (threadIdx.x < kC)
             partial[(int64_t)(tile * kC + threadIdx.x) * nblk + blockIdx.x] =
@@ -202,48 +225,47 @@ __global__ __launch_bounds__(kWG) void k_spmm_pw(
     const int *__restrict__ ntact_p, const float *__restrict__ P, float *__restrict__ Wv,
     float *__restrict__ partial)
 {
+    static_assert(kWG == 256, "one wavefront per slice, four per workgroup");
     __shared__ float red[(kWG / 64 + 1) * kC];
     const int ntact = *ntact_p;
     const int64_t nblk = gridDim.x;
     // plain block order (remapping blocks to give each XCD a contiguous row
     // range measured ~10 % slower for this gather)
-    const int64_t blk = blockIdx.x;
-    const int64_t row = blk * kWG + threadIdx.x;
+    const int64_t row = (int64_t)blockIdx.x * kWG + threadIdx.x;
     const bool live = row < n;
-    constexpr int WB = TWO ? W : 1;
-    int ca[W], cb[WB];
-    float va[W], vb[WB];
+    // the wavefront's slice width; every lane of a slice has stored entries
+    // (padding for rows past n), so all lanes gather and only live ones
+    // store
+    const int64_t s = wave_slice();
+    const int lane = (int)(threadIdx.x & 63);
     int w = 0;
-    if (live) {
-        const int64_t s = row >> 6;
-        const int lane = (int)(row & 63);
-        const int64_t base = slice_ptr[s];
-        w = (int)((slice_ptr[s + 1] - base) >> 6);
+    int64_t base = 0;
+    if (s < (n + 63) / 64) {  // wave-uniform
+        base = slice_ptr[s];
+        w = __builtin_amdgcn_readfirstlane((int)((slice_ptr[s + 1] - base) >> 6));
+    }
+    constexpr int WB = TWO ? W : 1;
+    uint32_t oa[W], ob[WB];  // byte offsets of the gathered 8-float rows
+    float va[W], vb[WB];
 #pragma unroll
-        for (int j = 0; j < W; ++j) {
-            if (j < w) {
-                const int64_t e = base + (int64_t)j * 64 + lane;
-                ca[j] = colA[e];
-                if constexpr (TWO) {
-                    va[j] = valA[e];
-                    vb[j] = sigma * valB[e];
-                    cb[j] = colB[e];
-                } else {
-                    va[j] = valB ? valA[e] + sigma * valB[e] : valA[e];
-                }
+    for (int j = 0; j < W; ++j) {
+        if (j < w) {  // wave-uniform
+            const int64_t e = base + (int64_t)j * 64 + lane;
+            oa[j] = (uint32_t)colA[e] * (uint32_t)(kC * sizeof(float));
+            if constexpr (TWO) {
+                va[j] = valA[e];
+                vb[j] = sigma * valB[e];
+                ob[j] = (uint32_t)colB[e] * (uint32_t)(kC * sizeof(float));
             } else {
-                ca[j] = (int)row;
-                va[j] = 0.f;
-                if constexpr (TWO) {
-                    cb[j] = (int)row;
-                    vb[j] = 0.f;
-                }
+                va[j] = valB ? valA[e] + sigma * valB[e] : valA[e];
             }
         }
     }
+    const uint32_t tbytes = (uint32_t)(n * kC * (int64_t)sizeof(float));
     for (int k = 0; k < ntact; ++k) {
         const int tile = tiles[k];
         const float *pt = P + (int64_t)tile * n * kC;
+        const auto pr_rsrc = buffer_rsrc(pt, tbytes);
         float dot[kC];
         f8 acc;
 #pragma unroll
@@ -251,20 +273,21 @@ __global__ __launch_bounds__(kWG) void k_spmm_pw(
             acc.v[t] = 0.f;
             dot[t] = 0.f;
         }
-        if (live) {
+        for_width<W>(w, [&](auto kw) {
+            constexpr int K = decltype(kw)::value;
 #pragma unroll
-            for (int j = 0; j < W; ++j) {
-                if (j < w) {
-                    const f8 pa = load8(pt + (int64_t)ca[j] * kC);
+            for (int j = 0; j < K; ++j) {
+                const f8 pa = buffer_load8(pr_rsrc, oa[j]);
 #pragma unroll
-                    for (int t = 0; t < kC; ++t) acc.v[t] += va[j] * pa.v[t];
-                    if constexpr (TWO) {
-                        const f8 pb = load8(pt + (int64_t)cb[j] * kC);
+                for (int t = 0; t < kC; ++t) acc.v[t] += va[j] * pa.v[t];
+                if constexpr (TWO) {
+                    const f8 pb = buffer_load8(pr_rsrc, ob[j]);
 #pragma unroll
-                        for (int t = 0; t < kC; ++t) acc.v[t] += vb[j] * pb.v[t];
-                    }
+                    for (int t = 0; t < kC; ++t) acc.v[t] += vb[j] * pb.v[t];
                 }
             }
+        });
+        if (live) {
             store8(Wv + ((int64_t)tile * n + row) * kC, acc);
             const f8 pr = load8(pt + row * kC);
 #pragma unroll
@@ -525,6 +548,9 @@ void pcg32_solve(void *matA, void **mv_b, void **mv_x, int *start, int *end, OPS
         sigma = (float)S.sigma;
     }
     const int64_t n = A->n;
+    // gathers use 32-bit byte offsets into one 8-column tile
+    if (n * kC * (int64_t)sizeof(float) > (int64_t)UINT32_MAX)
+        fail("BlockPCG32: more than 2^27 rows", __FILE__, __LINE__);
     S.ensure(n, nc);
     Ctx &c = ctx();
     hipStream_t st = c.stream;

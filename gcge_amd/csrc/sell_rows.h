@@ -1,0 +1,64 @@
+// Device helper for SELL-64 row kernels.
+//
+// A SELL-64 slice is 64 consecutive rows stored to the width of its widest
+// row, so with one thread per row a slice is exactly one wavefront and the
+// slice width is uniform across it.  Looping over a row's entries with a
+// per-entry guard (`for j < W: if (j < w) ...`) compiles to one branch per
+// entry, each ending in a wait for its own loads, which leaves a single
+// entry's gathers in flight per wavefront.  for_width instead turns the
+// wave-uniform width into a compile-time count: a scalar branch chain
+// selects a straight-line body with exactly w entries, whose gathers are
+// all issued before the first wait.  Entries are still summed in order
+// 0 .. w-1.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <type_traits>
+
+namespace gcge {
+
+// Calls f(std::integral_constant<int, K>{}) with K == w for 0 <= w <= W;
+// w must be the same on every active lane (pass it through
+// __builtin_amdgcn_readfirstlane).
+template <int W, int K = 0, typename F>
+__device__ __forceinline__ void for_width(int w, F &&f)
+{
+    if constexpr (K <= W) {
+        if (w == K)
+            f(std::integral_constant<int, K>{});
+        else
+            for_width<W, K + 1>(w, f);
+    }
+}
+
+// The slice of the calling wavefront for 256-thread workgroups over rows
+// in order: workgroup b, wavefront v holds slice 4 b + v.
+__device__ __forceinline__ int64_t wave_slice()
+{
+    return (int64_t)blockIdx.x * 4 + __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+}
+
+// Raw buffer loads: the base address and size live in a wave-uniform
+// resource (SGPRs) and each lane supplies a 32-bit byte offset, so a gather
+// costs one VGPR of address instead of two.  Offsets at or beyond the
+// resource's size read as zero.
+constexpr int kRsrcFlags = 0x00020000;  // raw 32-bit data, CDNA
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buffer_rsrc(const void *base, uint32_t bytes)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), (short)0, (int)bytes,
+                                             kRsrcFlags);
+}
+
+template <typename V>
+__device__ __forceinline__ V buffer_load(__amdgpu_buffer_rsrc_t r, uint32_t off)
+{
+    if constexpr (sizeof(V) == 8)
+        return __builtin_bit_cast(V, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
+    else
+        return __builtin_bit_cast(V, __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
+}
+
+}  // namespace gcge
