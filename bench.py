@@ -48,6 +48,19 @@ def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
+def tune_entries() -> int:
+    """Entries of the launch-shape table in use (the library loads the
+    table beside it when its first device context is created)."""
+    import ctypes as C
+    from gcge_amd import abi
+    lib = abi.load()
+    n = 0
+    z = [C.c_int() for _ in range(5)] + [C.c_double(), C.c_double()]
+    while lib.gcge_tune_entry(n, *[C.byref(x) for x in z]):
+        n += 1
+    return n
+
+
 def make_problem(name: str):
     from gcge_amd import problems as P
     if name == "cfg2":
@@ -361,6 +374,7 @@ def served(args):
         "solve_s_p50": float(np.percentile([r.solve_seconds for r in res], 50)),
         "roofline": None, "cpu_baseline": None,
         "cpu_baseline_note": "not measured for the batched/served configurations",
+        "tuning_entries": tune_entries(),
     }
     print(json.dumps(line))
 
@@ -379,10 +393,16 @@ def main():
     ap.add_argument("--qps", type=float, default=20.0, help="cfg5 arrival rate")
     ap.add_argument("--workers", type=int, default=8, help="cfg4/cfg5 service workers")
     ap.add_argument("--hbm-budget", type=float, default=200e9)
+    ap.add_argument("--tune-table", default=None,
+                    help="launch-shape table to load (default: the one beside the library; "
+                         "'none': built-in heuristics only)")
     ap.add_argument("--rocprof-summary", metavar="DIR")
     ap.add_argument("--pmc-traffic", nargs=5,
                     metavar=("FETCH", "WRITE", "COPY_FETCH", "COPY_WRITE", "BENCH_JSON"))
     args = ap.parse_args()
+    if args.tune_table is not None:
+        # read by the library when its first device context is created
+        os.environ["GCGE_TUNE_TABLE"] = "" if args.tune_table == "none" else args.tune_table
     if args.rocprof_summary:
         rocprof_summary(args.rocprof_summary)
         return
@@ -486,6 +506,7 @@ def main():
                   for m, r in results.items()},
         "upload_s": t_upload,
         "device_assembly": assembly,
+        "tuning_entries": tune_entries(),
         "roofline": roof,
         "cpu_baseline": cpu,
     }

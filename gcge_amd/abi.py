@@ -142,6 +142,14 @@ EXPORTS = {
     "HIPCCS_Create": (C.c_void_p, [C.POINTER(CCSMat), C.c_int]),
     "HIPCCS_Destroy": (None, [C.POINTER(C.c_void_p)]),
     "HIPCCS_CreateGrid": (C.c_void_p, [C.POINTER(GridOp), C.c_int]),
+    "gcge_tune_set_mode": (None, [C.c_int]),
+    "gcge_tune_get_mode": (C.c_int, []),
+    "gcge_tune_load": (C.c_int, [C.c_char_p]),
+    "gcge_tune_save": (C.c_int, [C.c_char_p]),
+    "gcge_tune_clear": (None, []),
+    "gcge_tune_set": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]),
+    "gcge_tune_entry": (C.c_int, [C.c_int, c_int_p, c_int_p, c_int_p, c_int_p, c_int_p,
+                                  c_dbl_p, c_dbl_p]),
     "HIPCCS_Equal": (C.c_int, [C.c_void_p, C.c_void_p]),
     "HIPCCS_Info": (None, [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64),
                            C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),

@@ -20,6 +20,7 @@
 #include "app_hip_internal.h"
 #include "gcge_hip.h"
 #include "kernels.h"
+#include "tuner.h"
 
 namespace gcge {
 
@@ -61,6 +62,7 @@ Ctx &ctx()
 {
     if (t_ctx == nullptr) {
         std::lock_guard<std::mutex> lk(g_ctx_mu);
+        tune_load_default();
         Ctx *x = new Ctx();
         HIPCHECK(hipGetDevice(&x->device));
         HIPCHECK(hipStreamCreateWithFlags(&x->stream, hipStreamNonBlocking));
@@ -500,7 +502,7 @@ void inner_prod_impl(char nsd, MV *X, MV *Y, int *start, int *end, double *ip, i
     }
     const bool sym = (nsd == 'S');
     if (sym && nr != nc) fail("MultiVecInnerProd 'S': block not square", __FILE__, __LINE__);
-    double *part = c.scratch(gram_scratch(X->nrows, nr, nc));
+    double *part = c.scratch(gram_scratch(X->nrows, nr, nc, sym));
     double *d = c.small((int64_t)nr * nc);
     hipEvent_t ev = c.prof.begin(PROF_GRAM, 8.0 * (double)X->nrows * (nr + nc),
                                  2.0 * (double)X->nrows * nr * nc, c.stream);

@@ -42,8 +42,8 @@ void col_dots(int64_t nrows, int ncols, const double *X, int64_t ldx, const doub
               int64_t ldy, double *out, double *partial, hipStream_t s);
 // G (m x n, ldg) = X^T Y with X: nrows x m, Y: nrows x n.  lower_only
 // computes only entries with row >= col (the 'S' case).  partial is scratch
-// of gram_scratch(m, n) doubles.
-int64_t gram_scratch(int64_t nrows, int m, int n);
+// of gram_scratch(nrows, m, n, lower_only) doubles.
+int64_t gram_scratch(int64_t nrows, int m, int n, bool lower_only);
 void gram(int64_t nrows, int m, int n, const double *X, int64_t ldx, const double *Y,
           int64_t ldy, double *G, int ldg, bool lower_only, double *partial,
           hipStream_t s);
@@ -54,7 +54,7 @@ void lincomb(int64_t nrows, int m, int n, const double *X, int64_t ldx, const do
              int ldct, int beta_mode, double beta_scalar, const double *beta, double *Y,
              int64_t ldy, hipStream_t s);
 // The matrix-core implementations behind gram() and lincomb().
-int64_t gram_mfma_scratch(int64_t nrows, int m, int n);
+int64_t gram_mfma_scratch(int64_t nrows, int m, int n, bool lower_only);
 void gram_mfma(int64_t nrows, int m, int n, const double *X, int64_t ldx, const double *Y,
                int64_t ldy, double *G, int ldg, bool lower_only, double *partial,
                hipStream_t s);

@@ -14,6 +14,7 @@
 
 #include "kernels.h"
 #include "sell_rows.h"
+#include "tuner.h"
 
 namespace gcge {
 namespace {
@@ -33,18 +34,18 @@ inline unsigned grid1d(int64_t n, int per_thread = 1)
 // The tile loop runs in a body compiled for the slice's exact width
 // (for_width), so all w x T gathers of a tile are in flight together.
 // ---------------------------------------------------------------------
-template <typename V, int W, int T>
-__global__ __launch_bounds__(kWG) void k_spmm_sell(
+template <typename V, int W, int T, int BS>
+__global__ __launch_bounds__(BS) void k_spmm_sell(
     int64_t nrows, const int64_t *__restrict__ slice_ptr, const int *__restrict__ col,
     const V *__restrict__ val, const V *__restrict__ X, int64_t ldx, V *__restrict__ Y,
     int64_t ldy, int ncols)
 {
-    static_assert(kWG == 256, "one wavefront per slice, four per workgroup");
+    static_assert(BS % 64 == 0, "one wavefront per slice");
     // plain block order (remapping blocks to give each XCD a contiguous row
     // range measured ~10 % slower for this gather)
-    const int64_t row = (int64_t)blockIdx.x * kWG + threadIdx.x;
+    const int64_t row = (int64_t)blockIdx.x * BS + threadIdx.x;
     if (row >= nrows) return;
-    const int64_t s = wave_slice();
+    const int64_t s = wave_slice<BS>();
     const int lane = (int)(threadIdx.x & 63);
     const int64_t base = slice_ptr[s];
     const int w = __builtin_amdgcn_readfirstlane((int)((slice_ptr[s + 1] - base) >> 6));
@@ -116,32 +117,68 @@ __global__ __launch_bounds__(kWG) void k_spmm_sell_wide(
     }
 }
 
-// Column tiles narrow as rows widen (W entries x T columns of gathers in
-// flight per thread), so registers stay near the same budget: 4 columns
-// for rows of up to 8 entries, 2 up to 16, 1 up to 32.
+// Candidate launch shapes per entry bound W, the first being the default:
+// column tiles narrow as rows widen (W entries x T columns of gathers in
+// flight per thread), so registers stay near the same budget -- 4 columns
+// for rows of up to 8 entries, 2 up to 16, 1 up to 32 -- in 256-thread
+// workgroups; the tuner may pick narrower tiles or 64-thread workgroups
+// (more workgroups for small matrices).  All give the same bits.
+template <typename V>
+using SpmmFn = void (*)(const SellMat &, const V *, const V *, int64_t, V *, int64_t, int,
+                        hipStream_t);
+
+template <typename V, int W, int T, int BS>
+void spmm_shape(const SellMat &A, const V *val, const V *X, int64_t ldx, V *Y, int64_t ldy,
+                int ncols, hipStream_t s)
+{
+    const unsigned g = (unsigned)((A.nrows + BS - 1) / BS);
+    k_spmm_sell<V, W, T, BS><<<g, BS, 0, s>>>(A.nrows, A.slice_ptr, A.col, val, X, ldx, Y, ldy,
+                                              ncols);
+}
+
+template <typename V>
+struct SpmmShapes {
+    static constexpr SpmmFn<V> w8[6] = {spmm_shape<V, 8, 4, 256>, spmm_shape<V, 8, 2, 256>,
+                                        spmm_shape<V, 8, 1, 256>, spmm_shape<V, 8, 4, 64>,
+                                        spmm_shape<V, 8, 2, 64>,  spmm_shape<V, 8, 1, 64>};
+    static constexpr SpmmFn<V> w16[4] = {spmm_shape<V, 16, 2, 256>, spmm_shape<V, 16, 1, 256>,
+                                         spmm_shape<V, 16, 2, 64>, spmm_shape<V, 16, 1, 64>};
+    static constexpr SpmmFn<V> w32[2] = {spmm_shape<V, 32, 1, 256>, spmm_shape<V, 32, 1, 64>};
+};
+
+int ncols_class(int ncols)
+{
+    return ncols <= 1 ? 0 : ncols <= 2 ? 1 : ncols <= 4 ? 2 : ncols <= 8 ? 3 : ncols <= 32 ? 4 : 5;
+}
+
 template <typename V>
 void launch_spmm(const SellMat &A, const V *val, const V *X, int64_t ldx, V *Y,
                  int64_t ldy, int ncols, hipStream_t s)
 {
     if (ncols <= 0 || A.nrows == 0) return;
-    const unsigned g = grid1d(A.nrows);
     // the register kernels gather by 32-bit byte offsets within a column
     const bool small = A.nrows * (int64_t)sizeof(V) <= (int64_t)UINT32_MAX;
-    if (!small)
-        k_spmm_sell_wide<V><<<g, kWG, 0, s>>>(A.nrows, A.slice_ptr, A.col, val, X, ldx, Y, ldy,
-                                              ncols);
-    else if (A.max_width <= 8)
-        k_spmm_sell<V, 8, 4><<<g, kWG, 0, s>>>(A.nrows, A.slice_ptr, A.col, val, X, ldx, Y,
-                                               ldy, ncols);
-    else if (A.max_width <= 16)
-        k_spmm_sell<V, 16, 2><<<g, kWG, 0, s>>>(A.nrows, A.slice_ptr, A.col, val, X, ldx, Y,
-                                                ldy, ncols);
-    else if (A.max_width <= 32)
-        k_spmm_sell<V, 32, 1><<<g, kWG, 0, s>>>(A.nrows, A.slice_ptr, A.col, val, X, ldx, Y,
-                                                ldy, ncols);
-    else
-        k_spmm_sell_wide<V><<<g, kWG, 0, s>>>(A.nrows, A.slice_ptr, A.col, val, X, ldx, Y,
-                                              ldy, ncols);
+    if (!small || A.max_width > 32) {
+        k_spmm_sell_wide<V><<<grid1d(A.nrows), kWG, 0, s>>>(A.nrows, A.slice_ptr, A.col, val, X,
+                                                            ldx, Y, ldy, ncols);
+        return;
+    }
+    const int W = A.max_width <= 8 ? 8 : A.max_width <= 16 ? 16 : 32;
+    const SpmmFn<V> *cand = W == 8 ? SpmmShapes<V>::w8 : W == 16 ? SpmmShapes<V>::w16
+                                                                 : SpmmShapes<V>::w32;
+    const int ncand = W == 8 ? 6 : W == 16 ? 4 : 2;
+    const TuneKey key{kTuneSpmm, W + (sizeof(V) == 4 ? 1000 : 0), ilog2_floor(A.nrows),
+                      ncols_class(ncols)};
+    int ch = tune_lookup(key);
+    if (ch < 0 || ch >= ncand) {
+        if (tune_enabled()) {
+            tune_run(key, ncand, 0,
+                     [&](int i) { cand[i](A, val, X, ldx, Y, ldy, ncols, s); }, s);
+            return;
+        }
+        ch = 0;
+    }
+    cand[ch](A, val, X, ldx, Y, ldy, ncols, s);
 }
 
 // ---------------------------------------------------------------------
@@ -459,9 +496,9 @@ void col_dots(int64_t nrows, int ncols, const double *X, int64_t ldx, const doub
 // the streaming kernel, with X and Y swapped when X is the thin side.
 static bool gram_is_thin(int m, int n) { return m <= 8 || n <= 8; }
 
-int64_t gram_scratch(int64_t nrows, int m, int n)
+int64_t gram_scratch(int64_t nrows, int m, int n, bool lower_only)
 {
-    if (!gram_is_thin(m, n)) return gram_mfma_scratch(nrows, m, n);
+    if (!gram_is_thin(m, n)) return gram_mfma_scratch(nrows, m, n, lower_only);
     ThinPlan p = (n <= 8) ? thin_plan(nrows, m, n) : thin_plan(nrows, n, m);
     return (int64_t)p.nchunk * p.mtiles * p.mt * p.nb;
 }

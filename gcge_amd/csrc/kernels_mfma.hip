@@ -10,7 +10,9 @@
 #include <hip/hip_runtime.h>
 #include <cstdint>
 
+#include "app_hip_internal.h"
 #include "kernels.h"
+#include "tuner.h"
 
 namespace gcge {
 namespace {
@@ -185,18 +187,19 @@ struct GramPlanM {
     int64_t rows_per_chunk;
 };
 
-GramPlanM plan(int64_t nrows, int m, int n)
+// Row chunking: at most max_wg workgroups in total (256 CUs, two resident
+// each, at the default 512: one more than that runs alone in a second
+// round and doubles the time) and at least min_rows rows (min_rows / 16
+// stages) per chunk, so small problems still spread over many workgroups
+// instead of a few walking all their rows.
+GramPlanM plan_with(int64_t nrows, int m, int n, int64_t min_rows, int64_t max_wg)
 {
     GramPlanM p;
     p.tm = (m + kGT - 1) / kGT;
     p.tn = (n + kGT - 1) / kGT;
     const int64_t tiles = (int64_t)p.tm * p.tn;
-    // at most two workgroups per CU in total (256 CUs x 2 resident): one
-    // more than that runs alone in a second round and doubles the time
-    int64_t want = 512 / tiles;
-    // at least 512 rows (32 stages) per chunk: small problems still spread
-    // over many workgroups instead of a few walking all their rows
-    int64_t cap = (nrows + 511) / 512;
+    int64_t want = max_wg / tiles;
+    int64_t cap = (nrows + min_rows - 1) / min_rows;
     int64_t nc = want < cap ? want : cap;
     if (nc < 1) nc = 1;
     int64_t rpc = (nrows + nc - 1) / nc;
@@ -205,6 +208,30 @@ GramPlanM plan(int64_t nrows, int m, int n)
     p.nchunk = (int)((nrows + rpc - 1) / rpc);
     if (p.nchunk < 1) p.nchunk = 1;
     return p;
+}
+
+// Tuner candidates (min_rows, max_wg); the default is index kGramDefault.
+constexpr int64_t kGramMinRows[5] = {64, 128, 256, 512, 1024};
+constexpr int64_t kGramMaxWg[2] = {512, 1024};
+constexpr int kGramCands = 10;
+constexpr int kGramDefault = 3 * 2 + 0;  // 512 rows, 512 workgroups
+
+GramPlanM plan_cand(int64_t nrows, int m, int n, int i)
+{
+    return plan_with(nrows, m, n, kGramMinRows[i / 2], kGramMaxWg[i % 2]);
+}
+
+TuneKey gram_key(int64_t nrows, int m, int n, bool lower_only)
+{
+    const int tiles = ((m + kGT - 1) / kGT) * ((n + kGT - 1) / kGT);
+    return TuneKey{kTuneGram, ilog2_floor(nrows), tiles < 64 ? tiles : 64, lower_only ? 1 : 0};
+}
+
+GramPlanM plan(int64_t nrows, int m, int n, bool lower_only)
+{
+    int ch = tune_lookup(gram_key(nrows, m, n, lower_only));
+    if (ch < 0 || ch >= kGramCands) ch = kGramDefault;
+    return plan_cand(nrows, m, n, ch);
 }
 
 // ---------------------------------------------------------------------
@@ -325,24 +352,58 @@ __global__ __launch_bounds__(kWG) void k_lincomb_mfma(
 
 }  // namespace
 
-int64_t gram_mfma_scratch(int64_t nrows, int m, int n)
+int64_t gram_mfma_scratch(int64_t nrows, int m, int n, bool lower_only)
 {
-    GramPlanM p = plan(nrows, m, n);
+    GramPlanM p = plan(nrows, m, n, lower_only);
     return (int64_t)p.nchunk * p.tm * p.tn * kGT * kGT;
 }
 
-void gram_mfma(int64_t nrows, int m, int n, const double *X, int64_t ldx, const double *Y,
-               int64_t ldy, double *G, int ldg, bool lower_only, double *partial,
-               hipStream_t s)
+namespace {
+
+void launch_gram(const GramPlanM &p, int64_t nrows, int m, int n, const double *X, int64_t ldx,
+                 const double *Y, int64_t ldy, double *G, int ldg, bool lower_only,
+                 double *partial, hipStream_t s)
 {
-    if (m <= 0 || n <= 0) return;
-    GramPlanM p = plan(nrows, m, n);
     const int64_t nblk = (int64_t)p.nchunk * p.tm * p.tn;
     k_gram_mfma<<<(unsigned)nblk, kWG, 0, s>>>(nrows, m, n, X, ldx, Y, ldy, p.tm, p.tn,
                                                p.rows_per_chunk, lower_only ? 1 : 0, partial);
     const int64_t total = (int64_t)p.tm * p.tn * kGT * kGT;
     k_gram_mfma_reduce<<<(unsigned)((total + kWG - 1) / kWG), kWG, 0, s>>>(
         m, n, p.tm, p.tn, p.nchunk, partial, G, ldg);
+}
+
+}  // namespace
+
+void gram_mfma(int64_t nrows, int m, int n, const double *X, int64_t ldx, const double *Y,
+               int64_t ldy, double *G, int ldg, bool lower_only, double *partial,
+               hipStream_t s)
+{
+    if (m <= 0 || n <= 0) return;
+    const TuneKey key = gram_key(nrows, m, n, lower_only);
+    const int ch = tune_lookup(key);
+    if (ch < 0 && tune_enabled()) {
+        // time every chunking on a private partial buffer large enough for
+        // all of them (the caller sized its buffer for the default plan)
+        int64_t need = 0;
+        for (int i = 0; i < kGramCands; ++i) {
+            const GramPlanM q = plan_cand(nrows, m, n, i);
+            const int64_t e = (int64_t)q.nchunk * q.tm * q.tn * kGT * kGT;
+            if (e > need) need = e;
+        }
+        double *buf = nullptr;
+        HIPCHECK(hipMalloc(reinterpret_cast<void **>(&buf), sizeof(double) * (size_t)need));
+        tune_run(key, kGramCands, kGramDefault,
+                 [&](int i) {
+                     launch_gram(plan_cand(nrows, m, n, i), nrows, m, n, X, ldx, Y, ldy, G, ldg,
+                                 lower_only, buf, s);
+                 },
+                 s);
+        HIPCHECK(hipStreamSynchronize(s));
+        HIPCHECK(hipFree(buf));
+        return;
+    }
+    launch_gram(plan(nrows, m, n, lower_only), nrows, m, n, X, ldx, Y, ldy, G, ldg, lower_only,
+                partial, s);
 }
 
 void lincomb_mfma(int64_t nrows, int m, int n, const double *X, int64_t ldx, const double *Ct,

@@ -33,11 +33,13 @@ __device__ __forceinline__ void for_width(int w, F &&f)
     }
 }
 
-// The slice of the calling wavefront for 256-thread workgroups over rows
-// in order: workgroup b, wavefront v holds slice 4 b + v.
+// The slice of the calling wavefront for BS-thread workgroups over rows in
+// order: workgroup b, wavefront v holds slice (BS / 64) b + v.
+template <int BS = 256>
 __device__ __forceinline__ int64_t wave_slice()
 {
-    return (int64_t)blockIdx.x * 4 + __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    return (int64_t)blockIdx.x * (BS / 64) +
+           __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
 }
 
 // Raw buffer loads: the base address and size live in a wave-uniform

@@ -1,0 +1,206 @@
+// Launch-shape tuner: the process-wide table, timing of candidates and the
+// text format of saved tables.
+#include "tuner.h"
+
+#include <dlfcn.h>
+
+#include <atomic>
+#include <cstdio>
+#include <cstdlib>
+#include <map>
+#include <mutex>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "app_hip_internal.h"
+#include "gcge_hip.h"
+
+namespace gcge {
+namespace {
+
+struct Entry {
+    int choice = 0;
+    double best_us = 0.0, default_us = 0.0;
+};
+
+using KeyT = std::tuple<int, int, int, int>;
+
+std::mutex g_mu;
+std::map<KeyT, Entry> g_table;
+std::atomic<int> g_mode{0};
+std::once_flag g_default_once;
+
+KeyT key_of(const TuneKey &k) { return KeyT(k.op, k.a, k.b, k.c); }
+
+int load_file(const char *path, bool replace)
+{
+    FILE *f = fopen(path, "r");
+    if (f == nullptr) return -1;
+    std::map<KeyT, Entry> got;
+    char line[256];
+    while (fgets(line, sizeof(line), f)) {
+        if (line[0] == '#' || line[0] == '\n') continue;
+        int op, a, b, c, ch;
+        double bu = 0.0, du = 0.0;
+        if (sscanf(line, "%d %d %d %d %d %lf %lf", &op, &a, &b, &c, &ch, &bu, &du) >= 5) {
+            Entry e;
+            e.choice = ch;
+            e.best_us = bu;
+            e.default_us = du;
+            got[KeyT(op, a, b, c)] = e;
+        }
+    }
+    fclose(f);
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (replace) g_table.clear();
+    for (auto &kv : got) g_table[kv.first] = kv.second;
+    return (int)got.size();
+}
+
+// The directory holding this library.
+std::string library_dir()
+{
+    Dl_info info;
+    if (dladdr(reinterpret_cast<void *>(&library_dir), &info) == 0 || info.dli_fname == nullptr)
+        return std::string();
+    std::string p(info.dli_fname);
+    const size_t slash = p.rfind('/');
+    return slash == std::string::npos ? std::string(".") : p.substr(0, slash);
+}
+
+}  // namespace
+
+int ilog2_floor(int64_t v)
+{
+    int r = 0;
+    while (v > 1) {
+        v >>= 1;
+        ++r;
+    }
+    return r;
+}
+
+int tune_lookup(const TuneKey &k)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    auto it = g_table.find(key_of(k));
+    return it == g_table.end() ? -1 : it->second.choice;
+}
+
+bool tune_enabled() { return g_mode.load() != 0; }
+
+int tune_run(const TuneKey &k, int ncand, int dflt, const std::function<void(int)> &launch,
+             hipStream_t s)
+{
+    constexpr int kReps = 3;
+    hipEvent_t e0, e1;
+    HIPCHECK(hipEventCreate(&e0));
+    HIPCHECK(hipEventCreate(&e1));
+    std::vector<double> us((size_t)ncand, 0.0);
+    for (int i = 0; i < ncand; ++i) {
+        launch(i);  // first launch of a candidate: code load, caches
+        HIPCHECK(hipEventRecord(e0, s));
+        for (int r = 0; r < kReps; ++r) launch(i);
+        HIPCHECK(hipEventRecord(e1, s));
+        HIPCHECK(hipEventSynchronize(e1));
+        float ms = 0.f;
+        HIPCHECK(hipEventElapsedTime(&ms, e0, e1));
+        us[(size_t)i] = 1e3 * ms / kReps;
+    }
+    int best = dflt;
+    for (int i = 0; i < ncand; ++i)
+        if (us[(size_t)i] < us[(size_t)best]) best = i;
+    // leave the outputs as the chosen candidate writes them
+    if (best != ncand - 1) launch(best);
+    HIPCHECK(hipEventDestroy(e0));
+    HIPCHECK(hipEventDestroy(e1));
+    Entry e;
+    e.choice = best;
+    e.best_us = us[(size_t)best];
+    e.default_us = us[(size_t)dflt];
+    std::lock_guard<std::mutex> lk(g_mu);
+    g_table[key_of(k)] = e;
+    return best;
+}
+
+void tune_load_default()
+{
+    std::call_once(g_default_once, [] {
+        const char *env = getenv("GCGE_TUNE_TABLE");
+        std::string path;
+        if (env != nullptr)
+            path = env;  // empty: no table
+        else
+            path = library_dir() + "/tuning_gfx950.txt";
+        if (!path.empty()) load_file(path.c_str(), false);
+    });
+}
+
+}  // namespace gcge
+
+using namespace gcge;
+
+extern "C" {
+
+void gcge_tune_set_mode(int on) { g_mode.store(on != 0 ? 1 : 0); }
+
+int gcge_tune_get_mode(void) { return g_mode.load(); }
+
+int gcge_tune_load(const char *path)
+{
+    if (path == nullptr) return -1;
+    return load_file(path, false);
+}
+
+int gcge_tune_save(const char *path)
+{
+    if (path == nullptr) return -1;
+    FILE *f = fopen(path, "w");
+    if (f == nullptr) return -1;
+    std::lock_guard<std::mutex> lk(g_mu);
+    fprintf(f, "# gcge_hip launch-shape table: op a b c choice best_us default_us\n");
+    fprintf(f, "# op 1 SpMM (a: entry bound, b: floor(log2 rows), c: column class)\n");
+    fprintf(f, "# op 2 Gram (a: floor(log2 rows), b: output tiles, c: 0)\n");
+    for (auto &kv : g_table)
+        fprintf(f, "%d %d %d %d %d %.3f %.3f\n", std::get<0>(kv.first), std::get<1>(kv.first),
+                std::get<2>(kv.first), std::get<3>(kv.first), kv.second.choice,
+                kv.second.best_us, kv.second.default_us);
+    fclose(f);
+    return (int)g_table.size();
+}
+
+void gcge_tune_clear(void)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    g_table.clear();
+}
+
+int gcge_tune_set(int op, int a, int b, int c, int choice)
+{
+    if (choice < 0) return -1;
+    std::lock_guard<std::mutex> lk(g_mu);
+    Entry e;
+    e.choice = choice;
+    g_table[KeyT(op, a, b, c)] = e;
+    return 0;
+}
+
+int gcge_tune_entry(int i, int *op, int *a, int *b, int *c, int *choice, double *best_us,
+                    double *default_us)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (i < 0 || i >= (int)g_table.size()) return 0;
+    auto it = g_table.begin();
+    std::advance(it, i);
+    if (op) *op = std::get<0>(it->first);
+    if (a) *a = std::get<1>(it->first);
+    if (b) *b = std::get<2>(it->first);
+    if (c) *c = std::get<3>(it->first);
+    if (choice) *choice = it->second.choice;
+    if (best_us) *best_us = it->second.best_us;
+    if (default_us) *default_us = it->second.default_us;
+    return 1;
+}
+
+}  // extern "C"

@@ -171,6 +171,28 @@ void gcge_hip_profile_reset(void);
 void gcge_hip_profile_read(int kind, int64_t *launches, double *total_ms, double *bytes,
                            double *flops);
 
+/* Launch-shape tuner (no counterpart in the reference, whose kernels are
+ * MKL's).  A process-wide table maps (operation, shape class) to a
+ * launch choice: op 1 the SpMM's column tile and workgroup size (all
+ * choices give the same bits), op 2 the MFMA Gram's row chunking (the
+ * chunking groups the partial sums, so Gram results are deterministic for
+ * a given table).  With tuning on, the first call of a class missing from
+ * the table times every candidate on its own operands and records the
+ * fastest; with it off, missing classes use the built-in heuristic.  The
+ * table tuned on MI355X (tuning_gfx950.txt beside the library, or the
+ * file named by GCGE_TUNE_TABLE; an empty value loads none) is loaded
+ * when the first device context is created.  gcge_tune_load merges a file
+ * into the table and gcge_tune_save writes it; both return the number of
+ * entries or -1.  gcge_tune_entry reads entry i (1 if it exists). */
+void gcge_tune_set_mode(int on);
+int gcge_tune_get_mode(void);
+int gcge_tune_load(const char *path);
+int gcge_tune_save(const char *path);
+void gcge_tune_clear(void);
+int gcge_tune_set(int op, int a, int b, int c, int choice);
+int gcge_tune_entry(int i, int *op, int *a, int *b, int *c, int *choice, double *best_us,
+                    double *default_us);
+
 /* ------------------------------------------------------------------ */
 /* The GCG driver                                                      */
 /* ------------------------------------------------------------------ */
