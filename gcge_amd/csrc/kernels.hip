@@ -34,18 +34,17 @@ inline unsigned grid1d(int64_t n, int per_thread = 1)
 // The tile loop runs in a body compiled for the slice's exact width
 // (for_width), so all w x T gathers of a tile are in flight together.
 // ---------------------------------------------------------------------
-template <typename V, int W, int T, int BS>
+template <typename V, int W, int T, int BS, bool XC>
 __global__ __launch_bounds__(BS) void k_spmm_sell(
     int64_t nrows, const int64_t *__restrict__ slice_ptr, const int *__restrict__ col,
     const V *__restrict__ val, const V *__restrict__ X, int64_t ldx, V *__restrict__ Y,
     int64_t ldy, int ncols)
 {
     static_assert(BS % 64 == 0, "one wavefront per slice");
-    // plain block order (remapping blocks to give each XCD a contiguous row
-    // range measured ~10 % slower for this gather)
-    const int64_t row = (int64_t)blockIdx.x * BS + threadIdx.x;
+    const int64_t rb = row_block<XC>();
+    const int64_t row = rb * BS + threadIdx.x;
     if (row >= nrows) return;
-    const int64_t s = wave_slice<BS>();
+    const int64_t s = wave_slice<BS>(rb);
     const int lane = (int)(threadIdx.x & 63);
     const int64_t base = slice_ptr[s];
     const int w = __builtin_amdgcn_readfirstlane((int)((slice_ptr[s + 1] - base) >> 6));
@@ -127,23 +126,32 @@ template <typename V>
 using SpmmFn = void (*)(const SellMat &, const V *, const V *, int64_t, V *, int64_t, int,
                         hipStream_t);
 
-template <typename V, int W, int T, int BS>
+template <typename V, int W, int T, int BS, bool XC>
 void spmm_shape(const SellMat &A, const V *val, const V *X, int64_t ldx, V *Y, int64_t ldy,
                 int ncols, hipStream_t s)
 {
     const unsigned g = (unsigned)((A.nrows + BS - 1) / BS);
-    k_spmm_sell<V, W, T, BS><<<g, BS, 0, s>>>(A.nrows, A.slice_ptr, A.col, val, X, ldx, Y, ldy,
-                                              ncols);
+    k_spmm_sell<V, W, T, BS, XC><<<g, BS, 0, s>>>(A.nrows, A.slice_ptr, A.col, val, X, ldx, Y,
+                                                  ldy, ncols);
 }
 
+// (the XCD-contiguous shapes come last so that table entries made before
+// they existed keep their meaning)
 template <typename V>
 struct SpmmShapes {
-    static constexpr SpmmFn<V> w8[6] = {spmm_shape<V, 8, 4, 256>, spmm_shape<V, 8, 2, 256>,
-                                        spmm_shape<V, 8, 1, 256>, spmm_shape<V, 8, 4, 64>,
-                                        spmm_shape<V, 8, 2, 64>,  spmm_shape<V, 8, 1, 64>};
-    static constexpr SpmmFn<V> w16[4] = {spmm_shape<V, 16, 2, 256>, spmm_shape<V, 16, 1, 256>,
-                                         spmm_shape<V, 16, 2, 64>, spmm_shape<V, 16, 1, 64>};
-    static constexpr SpmmFn<V> w32[2] = {spmm_shape<V, 32, 1, 256>, spmm_shape<V, 32, 1, 64>};
+    static constexpr SpmmFn<V> w8[9] = {
+        spmm_shape<V, 8, 4, 256, false>, spmm_shape<V, 8, 2, 256, false>,
+        spmm_shape<V, 8, 1, 256, false>, spmm_shape<V, 8, 4, 64, false>,
+        spmm_shape<V, 8, 2, 64, false>,  spmm_shape<V, 8, 1, 64, false>,
+        spmm_shape<V, 8, 4, 256, true>,  spmm_shape<V, 8, 2, 256, true>,
+        spmm_shape<V, 8, 1, 256, true>};
+    static constexpr SpmmFn<V> w16[6] = {
+        spmm_shape<V, 16, 2, 256, false>, spmm_shape<V, 16, 1, 256, false>,
+        spmm_shape<V, 16, 2, 64, false>,  spmm_shape<V, 16, 1, 64, false>,
+        spmm_shape<V, 16, 2, 256, true>,  spmm_shape<V, 16, 1, 256, true>};
+    static constexpr SpmmFn<V> w32[3] = {spmm_shape<V, 32, 1, 256, false>,
+                                         spmm_shape<V, 32, 1, 64, false>,
+                                         spmm_shape<V, 32, 1, 256, true>};
 };
 
 int ncols_class(int ncols)
@@ -166,7 +174,7 @@ void launch_spmm(const SellMat &A, const V *val, const V *X, int64_t ldx, V *Y,
     const int W = A.max_width <= 8 ? 8 : A.max_width <= 16 ? 16 : 32;
     const SpmmFn<V> *cand = W == 8 ? SpmmShapes<V>::w8 : W == 16 ? SpmmShapes<V>::w16
                                                                  : SpmmShapes<V>::w32;
-    const int ncand = W == 8 ? 6 : W == 16 ? 4 : 2;
+    const int ncand = W == 8 ? 9 : W == 16 ? 6 : 3;
     const TuneKey key{kTuneSpmm, W + (sizeof(V) == 4 ? 1000 : 0), ilog2_floor(A.nrows),
                       ncols_class(ncols)};
     int ch = tune_lookup(key);

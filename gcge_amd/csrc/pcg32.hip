@@ -38,6 +38,7 @@
 #include "gcge_hip.h"
 #include "kernels.h"
 #include "sell_rows.h"
+#include "tuner.h"
 
 namespace gcge {
 namespace {
@@ -111,7 +112,7 @@ __device__ __forceinline__ f8 buffer_load8(__amdgpu_buffer_rsrc_t r, uint32_t of
 // refinement step: the FP32 solve then finds the correction, whose size
 // is that of the residual, not the whole solution, whose FP32 rounding
 // would swamp the correction.
-template <int W, bool SHIFT>
+template <int W, bool SHIFT, bool XC>
 __global__ __launch_bounds__(kWG) void k_resid64(
     int64_t n, const int64_t *__restrict__ slice_ptr, const int *__restrict__ colA,
     const double *__restrict__ valA, const int *__restrict__ colB,
@@ -122,14 +123,15 @@ __global__ __launch_bounds__(kWG) void k_resid64(
     static_assert(kWG == 256, "one wavefront per slice, four per workgroup");
     __shared__ float red[(kWG / 64 + 1) * kC];
     const int64_t nblk = gridDim.x;
-    // plain block order (remapping blocks to give each XCD a contiguous row
-    // range measured ~10 % slower for this gather)
-    const int64_t row = (int64_t)blockIdx.x * kWG + threadIdx.x;
+    // partial sums are stored by row block, so their order (and the
+    // rounding of their sum) does not depend on XC
+    const int64_t rb = row_block<XC>();
+    const int64_t row = rb * kWG + threadIdx.x;
     const bool live = row < n;
     // the wavefront's slice width; every lane of a slice has stored entries
     // (padding for rows past n), so all lanes gather and only live ones
     // store
-    const int64_t s = wave_slice();
+    const int64_t s = wave_slice<kWG>(rb);
     const int lane = (int)(threadIdx.x & 63);
     int w = 0;
     int64_t base = 0;
@@ -183,7 +185,7 @@ __global__ __launch_bounds__(kWG) void k_resid64(
         if (live) store8(R + ((int64_t)tile * n + row) * kC, r8);
         block_sum8(sq, red);
         if (threadIdx.x < kC)
-            partial[(int64_t)(tile * kC + threadIdx.x) * nblk + blockIdx.x] =
+            partial[(int64_t)(tile * kC + threadIdx.x) * nblk + rb] =
                 red[kWG / 64 * kC + threadIdx.x];
         __syncthreads();
     }
@@ -217,7 +219,7 @@ __global__ __launch_bounds__(kWG) void k_zero_tiles(int64_t n, float *__restrict
 // partial[(tile*8 + t) * nblk + block] = the block's sum of p * w.
 // TWO: B is a matrix of its own (a shift by a B other than A); otherwise
 // the operator is A (+ sigma A) and only A's entries are held in registers.
-template <int W, bool TWO>
+template <int W, bool TWO, bool XC>
 __global__ __launch_bounds__(kWG) void k_spmm_pw(
     int64_t n, const int64_t *__restrict__ slice_ptr, const int *__restrict__ colA,
     const float *__restrict__ valA, const int *__restrict__ colB,
@@ -229,14 +231,15 @@ __global__ __launch_bounds__(kWG) void k_spmm_pw(
     __shared__ float red[(kWG / 64 + 1) * kC];
     const int ntact = *ntact_p;
     const int64_t nblk = gridDim.x;
-    // plain block order (remapping blocks to give each XCD a contiguous row
-    // range measured ~10 % slower for this gather)
-    const int64_t row = (int64_t)blockIdx.x * kWG + threadIdx.x;
+    // partial sums are stored by row block, so their order (and the
+    // rounding of their sum) does not depend on XC
+    const int64_t rb = row_block<XC>();
+    const int64_t row = rb * kWG + threadIdx.x;
     const bool live = row < n;
     // the wavefront's slice width; every lane of a slice has stored entries
     // (padding for rows past n), so all lanes gather and only live ones
     // store
-    const int64_t s = wave_slice();
+    const int64_t s = wave_slice<kWG>(rb);
     const int lane = (int)(threadIdx.x & 63);
     int w = 0;
     int64_t base = 0;
@@ -295,7 +298,7 @@ __global__ __launch_bounds__(kWG) void k_spmm_pw(
         }
         block_sum8(dot, red);
         if (threadIdx.x < kC)
-            partial[(int64_t)(tile * kC + threadIdx.x) * nblk + blockIdx.x] =
+            partial[(int64_t)(tile * kC + threadIdx.x) * nblk + rb] =
                 red[kWG / 64 * kC + threadIdx.x];
         __syncthreads();
     }
@@ -464,14 +467,20 @@ struct PCG32 {
     // kCheck inner iterations captured as one graph, for one key
     bool use_graphs = true;
     hipGraphExec_t gexec = nullptr;
+    // everything the captured launches take by value: the launch grids
+    // (active count, block width), the matrices, the shift, the stopping
+    // rule and the row-block order
     struct GraphKey {
         int ntact = -1, nc = 0;
-        const void *a = nullptr, *b = nullptr, *bv = nullptr;
-        float sigma = 0.f;
+        int64_t n = 0;
+        const void *a = nullptr, *as = nullptr, *av = nullptr, *b = nullptr, *bv = nullptr;
+        float sigma = 0.f, rate = 0.f, tol = 0.f;
+        bool xc = false;
         bool operator==(const GraphKey &o) const
         {
-            return ntact == o.ntact && nc == o.nc && a == o.a && b == o.b && bv == o.bv &&
-                   sigma == o.sigma;
+            return ntact == o.ntact && nc == o.nc && n == o.n && a == o.a && as == o.as &&
+                   av == o.av && b == o.b && bv == o.bv && sigma == o.sigma && rate == o.rate &&
+                   tol == o.tol && xc == o.xc;
         }
     } gkey;
     void drop_graph()
@@ -556,20 +565,38 @@ void pcg32_solve(void *matA, void **mv_b, void **mv_x, int *start, int *end, OPS
     hipStream_t st = c.stream;
     const int ntile = (nc + kC - 1) / kC;
     const unsigned nblk_row = (unsigned)((n + kWG - 1) / kWG);
+    // B's entries enter separately only for a shift by a B other than A
+    const bool two = sigma != 0.f && Bs->col != A->sell.col;
+    // row-block order of the gathering kernels (XCD-contiguous or plain):
+    // the table's choice, tuned below on the first iteration's operands
+    const TuneKey pkey{kTuneSpmmPw, (A->sell.max_width <= 8 ? 8 : 32) + (two ? 100 : 0),
+                       ilog2_floor(n), 0};
+    int pch = tune_lookup(pkey);
+    const bool tune_pw = (pch < 0 || pch > 1) && tune_enabled();
+    if (pch < 0 || pch > 1) pch = 0;
+    bool xc = pch == 1;
     k_zero_tiles<<<dim3(nblk_row, ntile), kWG, 0, st>>>(n, S.x);
     {
         const double *b64 = Bv->d + (int64_t)start[0] * Bv->ldd;
         const double *x64 = Xv->d + (int64_t)start[1] * Xv->ldd;
         const double *vB = (Bs != &A->sell) ? Bs->val : nullptr;
-#define RESID(W, SH)                                                                          \
-    k_resid64<W, SH><<<nblk_row, kWG, 0, st>>>(n, A->sell.slice_ptr, A->sell.col, A->sell.val,   \
-                                               Bs->col, vB, S.sigma, nc, b64, Bv->ldd, x64,      \
-                                               Xv->ldd, S.r, S.partial)
+#define RESID(W, SH, XC)                                                                      \
+    k_resid64<W, SH, XC><<<nblk_row, kWG, 0, st>>>(n, A->sell.slice_ptr, A->sell.col,            \
+                                                   A->sell.val, Bs->col, vB, S.sigma, nc, b64,   \
+                                                   Bv->ldd, x64, Xv->ldd, S.r, S.partial)
         const bool sh = vB != nullptr;
         if (A->sell.max_width <= 8) {
-            if (sh) RESID(8, true); else RESID(8, false);
+            if (sh) {
+                if (xc) RESID(8, true, true); else RESID(8, true, false);
+            } else {
+                if (xc) RESID(8, false, true); else RESID(8, false, false);
+            }
         } else {
-            if (sh) RESID(32, true); else RESID(32, false);
+            if (sh) {
+                if (xc) RESID(32, true, true); else RESID(32, true, false);
+            } else {
+                if (xc) RESID(32, false, true); else RESID(32, false, false);
+            }
         }
 #undef RESID
     }
@@ -590,9 +617,34 @@ void pcg32_solve(void *matA, void **mv_b, void **mv_x, int *start, int *end, OPS
     // r (and p after the first iteration) and writes p; d += alpha p,
     // r -= alpha w (with r.r) reads p, w, d, r and writes d, r
     const double tile_vec = 4.0 * kC * (double)n;
-    // B's entries enter separately only for a shift by a B other than A
-    const bool two = sigma != 0.f && Bs->col != A->sell.col;
     const float *vB = sigma != 0.f ? Bs->val32 : nullptr;
+    auto spmm_pw = [&](bool x) {
+#define SPMM_PW(W, TWO, XC)                                                                   \
+    k_spmm_pw<W, TWO, XC><<<nblk_row, kWG, 0, st>>>(n, A->sell.slice_ptr, A->sell.col,        \
+                                                    A->sell.val32, Bs->col, vB, sigma,         \
+                                                    S.tiles, S.ntact, S.p, S.w, S.partial)
+        if (A->sell.max_width <= 8) {
+            if (two) {
+                if (x) SPMM_PW(8, true, true); else SPMM_PW(8, true, false);
+            } else {
+                if (x) SPMM_PW(8, false, true); else SPMM_PW(8, false, false);
+            }
+        } else {
+            if (two) {
+                if (x) SPMM_PW(32, true, true); else SPMM_PW(32, true, false);
+            } else {
+                if (x) SPMM_PW(32, false, true); else SPMM_PW(32, false, false);
+            }
+        }
+#undef SPMM_PW
+    };
+    if (tune_pw && ntact > 0) {
+        // the tile list and count are set; p is written by the first
+        // iteration before it is read, and w and the partials are
+        // overwritten by every launch, so the timing runs leave no trace
+        const int best = tune_run(pkey, 2, 0, [&](int i) { spmm_pw(i == 1); }, st);
+        xc = best == 1;
+    }
     // one inner iteration on st; `prof` records the kernel-class events
     auto iteration = [&](int first, int nt, bool prof) {
         hipEvent_t evp = prof ? c.prof.begin(PROF_PCGVEC, (first ? 2.0 : 3.0) * tile_vec * nt,
@@ -604,16 +656,7 @@ void pcg32_solve(void *matA, void **mv_b, void **mv_x, int *start, int *end, OPS
         hipEvent_t ev = prof ? c.prof.begin(PROF_SPMM32, mat_bytes + 2.0 * 4.0 * kC * (double)nt * n,
                                             2.0 * (double)A->sell.nnz_true * kC * nt, st)
                              : nullptr;
-#define SPMM_PW(W, TWO)                                                                       \
-    k_spmm_pw<W, TWO><<<nblk_row, kWG, 0, st>>>(n, A->sell.slice_ptr, A->sell.col,            \
-                                                A->sell.val32, Bs->col, vB, sigma, S.tiles,    \
-                                                S.ntact, S.p, S.w, S.partial)
-        if (A->sell.max_width <= 8) {
-            if (two) SPMM_PW(8, true); else SPMM_PW(8, false);
-        } else {
-            if (two) SPMM_PW(32, true); else SPMM_PW(32, false);
-        }
-#undef SPMM_PW
+        spmm_pw(xc);
         if (prof) c.prof.end(ev, st);
         k_finish_pw<<<nt * kC, 64, 0, st>>>(S.tiles, S.ntact, (int)nblk_row, S.partial, S.rho,
                                             S.active, S.alpha);
@@ -640,10 +683,16 @@ void pcg32_solve(void *matA, void **mv_b, void **mv_x, int *start, int *end, OPS
         PCG32::GraphKey key;
         key.ntact = ntact;
         key.nc = nc;
+        key.n = n;
         key.a = A->sell.col;
+        key.as = A->sell.slice_ptr;
+        key.av = A->sell.val32;
         key.b = Bs->col;
         key.bv = vB;
         key.sigma = sigma;
+        key.rate = (float)S.rate;
+        key.tol = (float)S.tol;
+        key.xc = xc;
         const bool have = S.gexec && key == S.gkey;
         std::unique_lock<std::mutex> solo;
         if (whole && S.use_graphs && !c.prof.on && !have) solo = sole_context_lock();
@@ -708,6 +757,7 @@ extern "C" void HIPGCG_MultiLinearSolverSetup_BlockPCG32(int max_iter, double ra
     using namespace gcge;
     PCG32 *S = static_cast<PCG32 *>(ops->multi_linear_solver_workspace);
     if (S == nullptr || ops->MultiLinearSolver != pcg32_solve) S = new PCG32();
+    S->drop_graph();  // captured with the previous parameters
     S->max_iter = max_iter;
     S->rate = rate;
     S->tol = tol;

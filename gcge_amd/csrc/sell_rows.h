@@ -33,13 +33,33 @@ __device__ __forceinline__ void for_width(int w, F &&f)
     }
 }
 
-// The slice of the calling wavefront for BS-thread workgroups over rows in
-// order: workgroup b, wavefront v holds slice (BS / 64) b + v.
-template <int BS = 256>
-__device__ __forceinline__ int64_t wave_slice()
+// Row block of the calling workgroup.  The hardware places workgroup i on
+// XCD i % 8, each XCD with its own L2.  In plain order (XC false) block b
+// takes rows b BS ..., so neighbouring row blocks sit on different XCDs
+// and a gather of a neighbour row misses in this XCD's L2.  With XC each
+// XCD takes a contiguous range of row blocks, which keeps the rows a
+// stencil-like matrix gathers (near rows, and rows a grid plane away) in
+// the same L2.
+template <bool XC>
+__device__ __forceinline__ int64_t row_block()
 {
-    return (int64_t)blockIdx.x * (BS / 64) +
-           __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const int64_t bid = blockIdx.x;
+    if constexpr (!XC) {
+        return bid;
+    } else {
+        const int64_t nblk = gridDim.x;
+        const int64_t q = nblk / 8, r = nblk % 8;
+        const int64_t x = bid % 8, k = bid / 8;
+        return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + k;
+    }
+}
+
+// The slice of the calling wavefront for BS-thread workgroups: row block b,
+// wavefront v holds slice (BS / 64) b + v.
+template <int BS = 256>
+__device__ __forceinline__ int64_t wave_slice(int64_t b)
+{
+    return b * (BS / 64) + __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
 }
 
 // Raw buffer loads: the base address and size live in a wave-uniform

@@ -161,7 +161,9 @@ int gcge_tune_save(const char *path)
     std::lock_guard<std::mutex> lk(g_mu);
     fprintf(f, "# gcge_hip launch-shape table: op a b c choice best_us default_us\n");
     fprintf(f, "# op 1 SpMM (a: entry bound, b: floor(log2 rows), c: column class)\n");
-    fprintf(f, "# op 2 Gram (a: floor(log2 rows), b: output tiles, c: 0)\n");
+    fprintf(f, "# op 2 Gram (a: floor(log2 rows), b: output tiles, c: 'S' block)\n");
+    fprintf(f, "# op 3 FP32 BlockPCG SpMM (a: entry bound + 100 if B separate, b: floor(log2 "
+               "rows), c: 0)\n");
     for (auto &kv : g_table)
         fprintf(f, "%d %d %d %d %d %.3f %.3f\n", std::get<0>(kv.first), std::get<1>(kv.first),
                 std::get<2>(kv.first), std::get<3>(kv.first), kv.second.choice,

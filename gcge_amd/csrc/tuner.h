@@ -1,8 +1,10 @@
 // Launch-shape tuner.
 //
 // Some launch parameters have no single best value across problem shapes:
-// the SpMM's column tile and workgroup size, and the row chunking of the
-// MFMA Gram (how many workgroups share the rows of one output tile).  For
+// the SpMM's column tile, workgroup size and row-block order (plain or one
+// contiguous range per XCD), the row-block order of the FP32 BlockPCG's
+// gathering kernels, and the row chunking of the MFMA Gram (how many
+// workgroups share the rows of one output tile).  For
 // each such operation and shape class the tuner keeps a choice in a
 // process-wide table.  With tuning on, the first call of a class not yet
 // in the table times every candidate on the call's own operands (HIP events
@@ -12,8 +14,8 @@
 // on MI355X ships next to the library and is loaded when the first device
 // context is created.
 //
-// SpMM candidates give bit-identical results (each row is summed in the
-// same order).  Gram chunkings change the grouping of the partial sums, so
+// SpMM and BlockPCG candidates give bit-identical results (each row is
+// summed in the same order, partial sums are stored by row block).  Gram chunkings change the grouping of the partial sums, so
 // a Gram result depends on the table: results are deterministic for a
 // given table, and the shipped table fixes them by default.
 #pragma once
@@ -25,7 +27,7 @@
 
 namespace gcge {
 
-enum TuneOp : int { kTuneSpmm = 1, kTuneGram = 2 };
+enum TuneOp : int { kTuneSpmm = 1, kTuneGram = 2, kTuneSpmmPw = 3 };
 
 struct TuneKey {
     int op = 0, a = 0, b = 0, c = 0;

@@ -173,8 +173,9 @@ void gcge_hip_profile_read(int kind, int64_t *launches, double *total_ms, double
 
 /* Launch-shape tuner (no counterpart in the reference, whose kernels are
  * MKL's).  A process-wide table maps (operation, shape class) to a
- * launch choice: op 1 the SpMM's column tile and workgroup size (all
- * choices give the same bits), op 2 the MFMA Gram's row chunking (the
+ * launch choice: op 1 the SpMM's column tile, workgroup size and row-block
+ * order, op 3 the row-block order of the FP32 BlockPCG's gathering kernels
+ * (all these give the same bits), op 2 the MFMA Gram's row chunking (the
  * chunking groups the partial sums, so Gram results are deterministic for
  * a given table).  With tuning on, the first call of a class missing from
  * the table times every candidate on its own operands and records the

@@ -121,3 +121,71 @@ def test_graph_replays_match_eager(setup, rate):
     assert r.max() < (1.0 if rate == 0.0 else rate * 1.01)
     table.destroy_multivec(hb, nc)
     table.destroy_multivec(hx, nc)
+
+
+@pytest.mark.parametrize("shifted", [False, True])
+def test_row_block_orders_give_identical_bits(setup, tmp_path, shifted):
+    """Plain and XCD-contiguous row-block orders of the gathering kernels
+    (tuner op 3) give the same bits: partial sums are stored by row block."""
+    table, lib, A, B, dA, dB = setup
+    n, nc = A.shape[0], 12
+    keep = str(tmp_path / "kept.txt")
+    lib.gcge_tune_save(keep.encode())
+    rng = np.random.default_rng(43)
+    b = rng.standard_normal((n, nc))
+    hb = table.create_multivec(nc, dA)
+    hx = table.create_multivec(nc, dA)
+    table.upload(hb, b)
+    sigma = 0.5 if shifted else 0.0
+    # enough iterations for every column to reach the rate criterion
+    lib.HIPGCG_MultiLinearSolverSetup_BlockPCG32(400, 1e-2, 0.0,
+                                                 dB.handle if shifted else None, sigma,
+                                                 table.ops)
+    lg = int(np.floor(np.log2(n)))
+    outs = []
+    try:
+        for ch in (0, 1, 0, 1):
+            lib.gcge_tune_set(3, 8 + (100 if shifted else 0), lg, 0, ch)
+            table.upload(hx, np.zeros((n, nc)))
+            abi.slot(table.ops, "MultiLinearSolver")(dA.handle, hb, hx, ia(0, 0), ia(nc, nc),
+                                                     table.ops)
+            outs.append(table.download(hx, 0, nc, n))
+    finally:
+        lib.gcge_tune_clear()
+        lib.gcge_tune_load(keep.encode())
+        table.destroy_multivec(hb, nc)
+        table.destroy_multivec(hx, nc)
+    for o in outs[1:]:
+        assert np.array_equal(o, outs[0])
+    op = A + sigma * B
+    r = np.linalg.norm(b - op @ outs[0], axis=0) / np.linalg.norm(b, axis=0)
+    assert r.max() < 1.01e-2
+
+
+def test_graph_follows_new_parameters(setup):
+    """A graph captured under one stopping rule is not replayed under
+    another: after a new setup, graph and eager runs agree and the new
+    rate holds."""
+    table, lib, A, B, dA, dB = setup
+    n, nc = A.shape[0], 12
+    b = np.random.default_rng(44).standard_normal((n, nc))
+    hb = table.create_multivec(nc, dA)
+    hx = table.create_multivec(nc, dA)
+    table.upload(hb, b)
+    outs = {}
+    try:
+        for rate, graphs in ((0.0, 1), (1e-2, 1), (1e-2, 0)):
+            lib.HIPGCG_MultiLinearSolverSetup_BlockPCG32(300 if rate else 40, rate, 0.0, None,
+                                                         0.0, table.ops)
+            lib.HIPGCG_BlockPCG32_SetGraphs(table.ops, graphs)
+            table.upload(hx, np.zeros((n, nc)))
+            abi.slot(table.ops, "MultiLinearSolver")(dA.handle, hb, hx, ia(0, 0), ia(nc, nc),
+                                                     table.ops)
+            outs[(rate, graphs)] = table.download(hx, 0, nc, n)
+    finally:
+        lib.HIPGCG_BlockPCG32_SetGraphs(table.ops, 1)
+        table.destroy_multivec(hb, nc)
+        table.destroy_multivec(hx, nc)
+    assert np.array_equal(outs[(1e-2, 1)], outs[(1e-2, 0)])
+    r = np.linalg.norm(b - A @ outs[(1e-2, 1)], axis=0) / np.linalg.norm(b, axis=0)
+    assert r.max() < 1.01e-2
