@@ -754,6 +754,7 @@ void *HIPCCS_Create(const gcge_ccs_mat *host, int flags)
         HIPCHECK(hipMemcpy(S.val32, v32.data(), sizeof(float) * v32.size(), hipMemcpyHostToDevice));
         M->bytes += (int64_t)(sizeof(float) * v32.size());
     }
+    detect_plane(S, c.stream);
     (void)c;
     return M;
 }

@@ -337,6 +337,7 @@ void *HIPCCS_CreateGrid(const gcge_grid_op *op, int flags)
     }
     HIPCHECK(hipStreamSynchronize(st));  // the host coefficients and sp may go now
     if (tmp) HIPCHECK(hipFree(tmp));
+    detect_plane(S, st);
     return M;
 }
 

@@ -16,6 +16,12 @@ struct SellMat {
     int64_t nnz = 0;      // stored, including padding
     int64_t nnz_true = 0; // of the CCS input
     int max_width = 0;
+    // slices per grid plane: the typical reach max(col - row) of a row, in
+    // whole slices (0: under two slices, no plane structure to exploit)
+    int64_t plane = 0;
+    // in-plane reach: the typical largest |col - row| below half a plane
+    // (rows; 0 when unknown)
+    int64_t halo = 0;
     uint64_t structure_hash = 0;   // of the slice widths, to match A and B
     int64_t *slice_ptr = nullptr;  // nslices + 1
     int *col = nullptr;
@@ -23,6 +29,9 @@ struct SellMat {
     float *val32 = nullptr;
 };
 
+// Sets A.plane and A.halo from the offsets of a sample of rows
+// (stream-ordered, waits for the stream).
+void detect_plane(SellMat &A, hipStream_t s);
 // Y[:, c] = A X[:, c], c < ncols.
 void spmm(const SellMat &A, const double *X, int64_t ldx, double *Y, int64_t ldy,
           int ncols, hipStream_t s);

@@ -10,8 +10,11 @@
 // each XCD a contiguous row range instead measured ~10 % slower for the
 // SpMM gather).
 #include <hip/hip_runtime.h>
+#include <algorithm>
 #include <cstdint>
+#include <vector>
 
+#include "app_hip_internal.h"
 #include "kernels.h"
 #include "sell_rows.h"
 #include "tuner.h"
@@ -34,17 +37,30 @@ inline unsigned grid1d(int64_t n, int per_thread = 1)
 // The tile loop runs in a body compiled for the slice's exact width
 // (for_width), so all w x T gathers of a tile are in flight together.
 // ---------------------------------------------------------------------
-template <typename V, int W, int T, int BS, bool XC>
+// Row-block orders: plain, one contiguous range per XCD, plane-grouped,
+// plane-grouped over XCD-contiguous blocks; kLds: plain order with the X
+// window staged in LDS (k_spmm_sell_lds).
+enum RowOrder { kPlain = 0, kXcd = 1, kPlanes = 2, kPlanesXcd = 3, kLds = 4 };
+constexpr int kMaxHalo = 1024;
+
+template <typename V, int W, int T, int BS, int ORD>
 __global__ __launch_bounds__(BS) void k_spmm_sell(
     int64_t nrows, const int64_t *__restrict__ slice_ptr, const int *__restrict__ col,
     const V *__restrict__ val, const V *__restrict__ X, int64_t ldx, V *__restrict__ Y,
-    int64_t ldy, int ncols)
+    int64_t ldy, int ncols, int64_t plane)
 {
     static_assert(BS % 64 == 0, "one wavefront per slice");
-    const int64_t rb = row_block<XC>();
-    const int64_t row = rb * BS + threadIdx.x;
+    static_assert(ORD < kPlanes || BS == 256, "four planes per workgroup");
+    int64_t s;
+    if constexpr (ORD == kPlanes) {
+        s = plane_slice(plane, blockIdx.x);
+    } else if constexpr (ORD == kPlanesXcd) {
+        s = plane_slice(plane, row_block<true>());
+    } else {
+        s = wave_slice<BS>(row_block<ORD == kXcd>());
+    }
+    const int64_t row = s * 64 + (threadIdx.x & 63);
     if (row >= nrows) return;
-    const int64_t s = wave_slice<BS>(rb);
     const int lane = (int)(threadIdx.x & 63);
     const int64_t base = slice_ptr[s];
     const int w = __builtin_amdgcn_readfirstlane((int)((slice_ptr[s + 1] - base) >> 6));
@@ -89,6 +105,90 @@ __global__ __launch_bounds__(BS) void k_spmm_sell(
     });
 }
 
+// The same product with the X rows of the workgroup's 256 rows and `halo`
+// rows either side staged in LDS per tile of T columns (filled by coalesced
+// loads, column-major with one zero slot after each column).  An entry
+// whose column falls in the window reads LDS and gives its global load an
+// offset past the buffer (which reads zero without a memory access); one
+// outside reads the zero slot and loads from memory, so each operand is the
+// sum of the value and an exact zero and no branch splits the loads.  A
+// stencil's near neighbours then come from LDS instead of being fetched
+// again through L1 and L2 by each row that uses them.
+template <typename V, int W, int T>
+__global__ __launch_bounds__(256) void k_spmm_sell_lds(
+    int64_t nrows, const int64_t *__restrict__ slice_ptr, const int *__restrict__ col,
+    const V *__restrict__ val, const V *__restrict__ X, int64_t ldx, V *__restrict__ Y,
+    int64_t ldy, int ncols, int halo)
+{
+    extern __shared__ unsigned char lds_raw[];
+    V *win = reinterpret_cast<V *>(lds_raw);
+    const int span = 256 + 2 * halo;
+    const int ld = span + 1;  // the zero slot of column t is win[t * ld + span]
+    const int64_t r0 = (int64_t)blockIdx.x * 256;
+    const int64_t lo = r0 - halo;
+    const int64_t row = r0 + threadIdx.x;
+    const bool live = row < nrows;
+    const int64_t s = wave_slice<256>(blockIdx.x);
+    const int lane = (int)(threadIdx.x & 63);
+    int w = 0;
+    int64_t base = 0;
+    if (s < (nrows + 63) / 64) {  // wave-uniform
+        base = slice_ptr[s];
+        w = __builtin_amdgcn_readfirstlane((int)((slice_ptr[s + 1] - base) >> 6));
+    }
+    // a column's bytes: also the offset of the suppressed loads (exactly
+    // the resource's size, so every dword is past it; the host keeps
+    // colbytes + sizeof(V) below 2^32, so the check cannot wrap)
+    const uint32_t colbytes = (uint32_t)(nrows * (int64_t)sizeof(V));
+    uint32_t og[W];
+    int li[W];
+    V vj[W];
+#pragma unroll
+    for (int j = 0; j < W; ++j) {
+        if (j < w) {  // wave-uniform; the slice's padding covers every lane
+            const int64_t e = base + (int64_t)j * 64 + lane;
+            const int c = col[e];
+            vj[j] = val[e];
+            const int64_t d = (int64_t)c - lo;
+            const bool in = d >= 0 && d < span;
+            og[j] = in ? colbytes : (uint32_t)c * (uint32_t)sizeof(V);
+            li[j] = in ? (int)d : span;
+        }
+    }
+    if (threadIdx.x < T) win[threadIdx.x * ld + span] = V(0);
+    for (int c0 = 0; c0 < ncols; c0 += T) {
+        const int tn = ncols - c0 < T ? ncols - c0 : T;  // uniform
+        for (int i = threadIdx.x; i < span * tn; i += 256) {
+            const int t = i / span, rr = i - t * span;
+            const int64_t g = lo + rr;
+            win[t * ld + rr] = (g >= 0 && g < nrows) ? X[g + (int64_t)(c0 + t) * ldx] : V(0);
+        }
+        __syncthreads();
+        __amdgpu_buffer_rsrc_t rs[T];
+#pragma unroll
+        for (int t = 0; t < T; ++t)
+            rs[t] = buffer_rsrc(X + (int64_t)(c0 + (t < tn ? t : 0)) * ldx, colbytes);
+        for_width<W>(w, [&](auto kw) {
+            constexpr int K = decltype(kw)::value;
+            V acc[T];
+#pragma unroll
+            for (int t = 0; t < T; ++t) acc[t] = V(0);
+#pragma unroll
+            for (int j = 0; j < K; ++j) {
+#pragma unroll
+                for (int t = 0; t < T; ++t)
+                    acc[t] += vj[j] * (win[t * ld + li[j]] + buffer_load<V>(rs[t], og[j]));
+            }
+            if (live) {
+#pragma unroll
+                for (int t = 0; t < T; ++t)
+                    if (t < tn) Y[row + (int64_t)(c0 + t) * ldy] = acc[t];
+            }
+        });
+        __syncthreads();
+    }
+}
+
 // Rows of any width: entries re-read from memory for each column.
 template <typename V>
 __global__ __launch_bounds__(kWG) void k_spmm_sell_wide(
@@ -126,32 +226,61 @@ template <typename V>
 using SpmmFn = void (*)(const SellMat &, const V *, const V *, int64_t, V *, int64_t, int,
                         hipStream_t);
 
-template <typename V, int W, int T, int BS, bool XC>
+template <typename V, int W, int T, int BS, int ORD>
 void spmm_shape(const SellMat &A, const V *val, const V *X, int64_t ldx, V *Y, int64_t ldy,
                 int ncols, hipStream_t s)
 {
-    const unsigned g = (unsigned)((A.nrows + BS - 1) / BS);
-    k_spmm_sell<V, W, T, BS, XC><<<g, BS, 0, s>>>(A.nrows, A.slice_ptr, A.col, val, X, ldx, Y,
-                                                  ldy, ncols);
+    if constexpr (ORD == kLds) {
+        if (A.halo > 0 && A.halo <= kMaxHalo) {
+            const int halo = (int)A.halo;
+            const size_t lds = (size_t)(256 + 2 * halo + 1) * T * sizeof(V);
+            k_spmm_sell_lds<V, W, T><<<(unsigned)((A.nrows + 255) / 256), 256, lds, s>>>(
+                A.nrows, A.slice_ptr, A.col, val, X, ldx, Y, ldy, ncols, halo);
+            return;
+        }
+        k_spmm_sell<V, W, T, 256, kPlain><<<(unsigned)((A.nrows + 255) / 256), 256, 0, s>>>(
+            A.nrows, A.slice_ptr, A.col, val, X, ldx, Y, ldy, ncols, 0);
+    } else if constexpr (ORD >= kPlanes) {
+        if (A.plane > 0) {
+            const int64_t groups = (A.nslices + 4 * A.plane - 1) / (4 * A.plane);
+            k_spmm_sell<V, W, T, 256, ORD><<<(unsigned)(groups * A.plane), 256, 0, s>>>(
+                A.nrows, A.slice_ptr, A.col, val, X, ldx, Y, ldy, ncols, A.plane);
+            return;
+        }
+        // no plane structure: plain order
+        k_spmm_sell<V, W, T, 256, kPlain><<<(unsigned)((A.nrows + 255) / 256), 256, 0, s>>>(
+            A.nrows, A.slice_ptr, A.col, val, X, ldx, Y, ldy, ncols, 0);
+    } else {
+        const unsigned g = (unsigned)((A.nrows + BS - 1) / BS);
+        k_spmm_sell<V, W, T, BS, ORD><<<g, BS, 0, s>>>(A.nrows, A.slice_ptr, A.col, val, X, ldx,
+                                                       Y, ldy, ncols, 0);
+    }
 }
 
-// (the XCD-contiguous shapes come last so that table entries made before
-// they existed keep their meaning)
+// (shapes added later come last so that table entries made before they
+// existed keep their meaning)
 template <typename V>
 struct SpmmShapes {
-    static constexpr SpmmFn<V> w8[9] = {
-        spmm_shape<V, 8, 4, 256, false>, spmm_shape<V, 8, 2, 256, false>,
-        spmm_shape<V, 8, 1, 256, false>, spmm_shape<V, 8, 4, 64, false>,
-        spmm_shape<V, 8, 2, 64, false>,  spmm_shape<V, 8, 1, 64, false>,
-        spmm_shape<V, 8, 4, 256, true>,  spmm_shape<V, 8, 2, 256, true>,
-        spmm_shape<V, 8, 1, 256, true>};
-    static constexpr SpmmFn<V> w16[6] = {
-        spmm_shape<V, 16, 2, 256, false>, spmm_shape<V, 16, 1, 256, false>,
-        spmm_shape<V, 16, 2, 64, false>,  spmm_shape<V, 16, 1, 64, false>,
-        spmm_shape<V, 16, 2, 256, true>,  spmm_shape<V, 16, 1, 256, true>};
-    static constexpr SpmmFn<V> w32[3] = {spmm_shape<V, 32, 1, 256, false>,
-                                         spmm_shape<V, 32, 1, 64, false>,
-                                         spmm_shape<V, 32, 1, 256, true>};
+    static constexpr SpmmFn<V> w8[15] = {
+        spmm_shape<V, 8, 4, 256, kPlain>, spmm_shape<V, 8, 2, 256, kPlain>,
+        spmm_shape<V, 8, 1, 256, kPlain>, spmm_shape<V, 8, 4, 64, kPlain>,
+        spmm_shape<V, 8, 2, 64, kPlain>,  spmm_shape<V, 8, 1, 64, kPlain>,
+        spmm_shape<V, 8, 4, 256, kXcd>,   spmm_shape<V, 8, 2, 256, kXcd>,
+        spmm_shape<V, 8, 1, 256, kXcd>,   spmm_shape<V, 8, 4, 256, kPlanes>,
+        spmm_shape<V, 8, 1, 256, kPlanes>, spmm_shape<V, 8, 4, 256, kPlanesXcd>,
+        spmm_shape<V, 8, 1, 256, kPlanesXcd>, spmm_shape<V, 8, 4, 256, kLds>,
+        spmm_shape<V, 8, 1, 256, kLds>};
+    static constexpr SpmmFn<V> w16[11] = {
+        spmm_shape<V, 16, 2, 256, kPlain>, spmm_shape<V, 16, 1, 256, kPlain>,
+        spmm_shape<V, 16, 2, 64, kPlain>,  spmm_shape<V, 16, 1, 64, kPlain>,
+        spmm_shape<V, 16, 2, 256, kXcd>,   spmm_shape<V, 16, 1, 256, kXcd>,
+        spmm_shape<V, 16, 2, 256, kPlanes>, spmm_shape<V, 16, 1, 256, kPlanes>,
+        spmm_shape<V, 16, 1, 256, kPlanesXcd>, spmm_shape<V, 16, 2, 256, kLds>,
+        spmm_shape<V, 16, 1, 256, kLds>};
+    static constexpr SpmmFn<V> w32[6] = {
+        spmm_shape<V, 32, 1, 256, kPlain>, spmm_shape<V, 32, 1, 64, kPlain>,
+        spmm_shape<V, 32, 1, 256, kXcd>, spmm_shape<V, 32, 1, 256, kPlanes>,
+        spmm_shape<V, 32, 1, 256, kPlanesXcd>, spmm_shape<V, 32, 1, 256, kLds>};
 };
 
 int ncols_class(int ncols)
@@ -165,7 +294,7 @@ void launch_spmm(const SellMat &A, const V *val, const V *X, int64_t ldx, V *Y,
 {
     if (ncols <= 0 || A.nrows == 0) return;
     // the register kernels gather by 32-bit byte offsets within a column
-    const bool small = A.nrows * (int64_t)sizeof(V) <= (int64_t)UINT32_MAX;
+    const bool small = A.nrows * (int64_t)sizeof(V) <= (int64_t)UINT32_MAX - 64;
     if (!small || A.max_width > 32) {
         k_spmm_sell_wide<V><<<grid1d(A.nrows), kWG, 0, s>>>(A.nrows, A.slice_ptr, A.col, val, X,
                                                             ldx, Y, ldy, ncols);
@@ -174,7 +303,7 @@ void launch_spmm(const SellMat &A, const V *val, const V *X, int64_t ldx, V *Y,
     const int W = A.max_width <= 8 ? 8 : A.max_width <= 16 ? 16 : 32;
     const SpmmFn<V> *cand = W == 8 ? SpmmShapes<V>::w8 : W == 16 ? SpmmShapes<V>::w16
                                                                  : SpmmShapes<V>::w32;
-    const int ncand = W == 8 ? 9 : W == 16 ? 6 : 3;
+    const int ncand = W == 8 ? 15 : W == 16 ? 11 : 6;
     const TuneKey key{kTuneSpmm, W + (sizeof(V) == 4 ? 1000 : 0), ilog2_floor(A.nrows),
                       ncols_class(ncols)};
     int ch = tune_lookup(key);
@@ -454,6 +583,64 @@ __global__ __launch_bounds__(kWG) void k_to_f64(int64_t nrows, const float *__re
 }
 
 }  // namespace
+
+namespace {
+
+// off[k][j] = col - row of entry j < 32 of sampled row k (zero past the
+// row's width, which the padding also gives)
+__global__ __launch_bounds__(kWG) void k_row_offsets(int64_t nrows, int nsample,
+                                                     const int64_t *__restrict__ slice_ptr,
+                                                     const int *__restrict__ col,
+                                                     int64_t *__restrict__ off)
+{
+    const int k = blockIdx.x * kWG + threadIdx.x;
+    if (k >= nsample) return;
+    const int64_t r = (int64_t)k * nrows / nsample;
+    const int64_t s = r >> 6;
+    const int64_t base = slice_ptr[s];
+    const int w = (int)((slice_ptr[s + 1] - base) >> 6);
+    for (int j = 0; j < 32; ++j)
+        off[(int64_t)k * 32 + j] =
+            j < w ? (int64_t)col[base + (int64_t)j * 64 + (r & 63)] - r : 0;
+}
+
+}  // namespace
+
+void detect_plane(SellMat &A, hipStream_t s)
+{
+    A.plane = 0;
+    A.halo = 0;
+    if (A.nrows < 4096 || A.max_width > 32) return;
+    constexpr int kSample = 1024;
+    int64_t *d = nullptr;
+    HIPCHECK(hipMalloc(reinterpret_cast<void **>(&d), sizeof(int64_t) * kSample * 32));
+    k_row_offsets<<<(kSample + kWG - 1) / kWG, kWG, 0, s>>>(A.nrows, kSample, A.slice_ptr,
+                                                            A.col, d);
+    std::vector<int64_t> h((size_t)kSample * 32);
+    HIPCHECK(hipMemcpyAsync(h.data(), d, sizeof(int64_t) * h.size(), hipMemcpyDeviceToHost, s));
+    HIPCHECK(hipStreamSynchronize(s));
+    HIPCHECK(hipFree(d));
+    // plane: median over rows of the largest offset
+    std::vector<int64_t> m(kSample);
+    for (int k = 0; k < kSample; ++k)
+        m[k] = *std::max_element(h.begin() + (int64_t)k * 32, h.begin() + (int64_t)(k + 1) * 32);
+    std::vector<int64_t> mm = m;
+    std::nth_element(mm.begin(), mm.begin() + kSample / 2, mm.end());
+    const int64_t reach = mm[kSample / 2];
+    A.plane = reach / 64 >= 2 ? reach / 64 : 0;
+    // halo: median over rows of the largest |offset| below half the reach
+    for (int k = 0; k < kSample; ++k) {
+        int64_t b = 0;
+        for (int j = 0; j < 32; ++j) {
+            const int64_t v = h[(size_t)k * 32 + j];
+            const int64_t o = v < 0 ? -v : v;
+            if (2 * o < reach && o > b) b = o;
+        }
+        m[k] = b;
+    }
+    std::nth_element(m.begin(), m.begin() + kSample / 2, m.end());
+    A.halo = m[kSample / 2];
+}
 
 void spmm(const SellMat &A, const double *X, int64_t ldx, double *Y, int64_t ldy, int ncols,
           hipStream_t s)

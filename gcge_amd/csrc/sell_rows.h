@@ -54,6 +54,20 @@ __device__ __forceinline__ int64_t row_block()
     }
 }
 
+// Plane-grouped slices for 256-thread workgroups: with P slices per grid
+// plane, workgroup b takes in-plane slice j = b % P of four consecutive
+// planes 4 (b / P) .. +3, one per wavefront, so a row's neighbours one
+// plane away are mostly rows a sibling wavefront of the same CU gathers at
+// the same time.  The grid has ceil(S / 4P) P workgroups for S slices;
+// slices past S belong to no row and their wavefronts exit.
+// b is the row block (blockIdx.x, or its XCD-contiguous remap, which keeps
+// the in-plane neighbour workgroups b +- 1, b +- 2 on one XCD).
+__device__ __forceinline__ int64_t plane_slice(int64_t P, int64_t b)
+{
+    const int64_t w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    return (b / P) * 4 * P + w * P + b % P;
+}
+
 // The slice of the calling wavefront for BS-thread workgroups: row block b,
 // wavefront v holds slice (BS / 64) b + v.
 template <int BS = 256>

@@ -131,7 +131,7 @@ def solve(A, B, nev: int, block_size: int, nev_init: int, nev_max: int,
           max_iter: int = 1000, mode: str = "fp64", rand: str = "libc",
           seed: int = 0, eig: str = "auto", want_vectors: bool = False,
           print_level: int = 0, mats=None, orth: str | None = None,
-          graphs: bool = True) -> SolveResult:
+          graphs: bool = True, persistent: bool = False) -> SolveResult:
     """Smallest `nev` eigenpairs of A x = lambda B x on the GPU.
 
     mode "fp64": the reference-equivalent path, every operation in FP64.
@@ -144,6 +144,8 @@ def solve(A, B, nev: int, block_size: int, nev_init: int, nev_max: int,
     Cholesky QR); default "mgs" for fp64 and "cholqr" for mixed.
     graphs: replay the mixed path's BlockPCG iteration groups as captured
     HIP graphs (False launches every kernel eagerly).
+    persistent: run small systems' BlockPCG loops in one cooperative
+    launch (default False: one launch per step, which is faster on MI355X).
     `mats` may pass pre-uploaded (DeviceMatrix A, DeviceMatrix B).
     """
     lib = abi.load()
@@ -185,6 +187,7 @@ def solve(A, B, nev: int, block_size: int, nev_init: int, nev_max: int,
         lib.HIPGCG_MultiLinearSolverSetup_BlockPCG32(30, 1e-2, 1e-14, hB, 0.0, ops)
         if not graphs:
             lib.HIPGCG_BlockPCG32_SetGraphs(ops, 0)
+        lib.HIPGCG_BlockPCG32_SetPersistent(ops, 1 if persistent else 0)
     ev = np.zeros(nev_max)
     nconv = C.c_int(nev)
     lib.gcge_hip_synchronize()

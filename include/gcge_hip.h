@@ -288,6 +288,16 @@ void HIPGCG_MultiLinearSolverFree_BlockPCG32(OPS *ops);
  * captured HIP graph (default on; off while kernel timing is enabled).  No
  * counterpart in the reference; for callers that need plain stream launches. */
 void HIPGCG_BlockPCG32_SetGraphs(OPS *ops, int on);
+/* With on != 0, systems of at most 65,536 rows with rows of at most 8
+ * entries and at most 32 columns run the whole BlockPCG loop in one
+ * cooperative launch with grid-wide barriers (default off; never while
+ * kernel timing is enabled or while the process has more than one library
+ * context).  It is slower than the launch-per-step loop on MI355X.  A
+ * barrier wait that gives up, or a refused launch, makes the call solve
+ * again with one launch per step.  Counts: calls that ran in one launch,
+ * calls that fell back.  No counterpart in the reference. */
+void HIPGCG_BlockPCG32_SetPersistent(OPS *ops, int on);
+void HIPGCG_BlockPCG32_Counts(OPS *ops, int64_t *persistent, int64_t *fallbacks);
 
 /* ------------------------------------------------------------------ */
 /* Solve service: concurrent requests on one GPU                       */

@@ -56,11 +56,11 @@ def nine_point(n):
 @pytest.mark.parametrize("kind", ["7pt", "9pt", "27pt"])
 def test_spmm_shapes_give_identical_bits(lib, kind):
     if kind == "7pt":
-        A, W, ncand = P.stencil_pair(30, 3)[0], 8, 9
+        A, W, ncand = P.stencil_pair(30, 3)[0], 8, 15
     elif kind == "9pt":
-        A, W, ncand = nine_point(150), 16, 6
+        A, W, ncand = nine_point(150), 16, 11
     else:
-        A, W, ncand = P.config3_grid(14).A.to_ccs(), 32, 3
+        A, W, ncand = P.config3_grid(18).A.to_ccs(), 32, 6
     n = A.shape[0]
     table = HipTable()
     dA = DeviceMatrix(A)
@@ -94,9 +94,10 @@ def test_tuned_solve_is_recorded_and_reproduced(lib, tmp_path):
     lib.gcge_tune_set_mode(0)
     got = entries(lib)
     ops = {e[0] for e in got}
-    assert ops == {1, 2, 3}, got
+    # op 3 only when the BlockPCG runs one launch per step (larger systems)
+    assert {1, 2} <= ops <= {1, 2, 3}, got
     for op, a, b, c, ch, best, dflt in got:
-        assert 0 <= ch < {1: 9, 2: 10, 3: 2}[op]
+        assert 0 <= ch < {1: 15, 2: 10, 3: 2}[op]
         assert best <= dflt          # the default is among the candidates
     path = str(tmp_path / "table.txt")
     assert lib.gcge_tune_save(path.encode()) == len(got)
