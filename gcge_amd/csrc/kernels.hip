@@ -107,13 +107,12 @@ __global__ __launch_bounds__(BS) void k_spmm_sell(
 
 // The same product with the X rows of the workgroup's 256 rows and `halo`
 // rows either side staged in LDS per tile of T columns (filled by coalesced
-// loads, column-major with one zero slot after each column).  An entry
-// whose column falls in the window reads LDS and gives its global load an
-// offset past the buffer (which reads zero without a memory access); one
-// outside reads the zero slot and loads from memory, so each operand is the
-// sum of the value and an exact zero and no branch splits the loads.  A
-// stencil's near neighbours then come from LDS instead of being fetched
-// again through L1 and L2 by each row that uses them.
+// loads, column-major, one spare slot per column).  Entries whose column
+// falls outside the window are loaded from memory first, in a lane-masked
+// region whose results are not used inside it (lanes with an in-window
+// entry issue no request, and a stencil's interior slices skip the region
+// as a whole); then every entry takes its value from LDS or from the
+// loaded register, in entry order, so the sums are those of k_spmm_sell.
 template <typename V, int W, int T>
 __global__ __launch_bounds__(256) void k_spmm_sell_lds(
     int64_t nrows, const int64_t *__restrict__ slice_ptr, const int *__restrict__ col,
@@ -123,7 +122,7 @@ __global__ __launch_bounds__(256) void k_spmm_sell_lds(
     extern __shared__ unsigned char lds_raw[];
     V *win = reinterpret_cast<V *>(lds_raw);
     const int span = 256 + 2 * halo;
-    const int ld = span + 1;  // the zero slot of column t is win[t * ld + span]
+    const int ld = span + 1;  // slot span of each column: read by out-of-window entries
     const int64_t r0 = (int64_t)blockIdx.x * 256;
     const int64_t lo = r0 - halo;
     const int64_t row = r0 + threadIdx.x;
@@ -136,12 +135,10 @@ __global__ __launch_bounds__(256) void k_spmm_sell_lds(
         base = slice_ptr[s];
         w = __builtin_amdgcn_readfirstlane((int)((slice_ptr[s + 1] - base) >> 6));
     }
-    // a column's bytes: also the offset of the suppressed loads (exactly
-    // the resource's size, so every dword is past it; the host keeps
-    // colbytes + sizeof(V) below 2^32, so the check cannot wrap)
     const uint32_t colbytes = (uint32_t)(nrows * (int64_t)sizeof(V));
     uint32_t og[W];
     int li[W];
+    bool nr[W];
     V vj[W];
 #pragma unroll
     for (int j = 0; j < W; ++j) {
@@ -150,12 +147,11 @@ __global__ __launch_bounds__(256) void k_spmm_sell_lds(
             const int c = col[e];
             vj[j] = val[e];
             const int64_t d = (int64_t)c - lo;
-            const bool in = d >= 0 && d < span;
-            og[j] = in ? colbytes : (uint32_t)c * (uint32_t)sizeof(V);
-            li[j] = in ? (int)d : span;
+            nr[j] = d >= 0 && d < span;
+            og[j] = (uint32_t)c * (uint32_t)sizeof(V);
+            li[j] = nr[j] ? (int)d : span;
         }
     }
-    if (threadIdx.x < T) win[threadIdx.x * ld + span] = V(0);
     for (int c0 = 0; c0 < ncols; c0 += T) {
         const int tn = ncols - c0 < T ? ncols - c0 : T;  // uniform
         for (int i = threadIdx.x; i < span * tn; i += 256) {
@@ -170,14 +166,33 @@ __global__ __launch_bounds__(256) void k_spmm_sell_lds(
             rs[t] = buffer_rsrc(X + (int64_t)(c0 + (t < tn ? t : 0)) * ldx, colbytes);
         for_width<W>(w, [&](auto kw) {
             constexpr int K = decltype(kw)::value;
+            V xg[K > 0 ? K : 1][T];
+#pragma unroll
+            for (int j = 0; j < K; ++j) {
+                if (!nr[j]) {
+#pragma unroll
+                    for (int t = 0; t < T; ++t) xg[j][t] = buffer_load<V>(rs[t], og[j]);
+                }
+            }
+            // every LDS read issued before any is consumed; the empty asm
+            // statements keep them unconditional, so the choice below is a
+            // select rather than a branch around each read
+            V xl[K > 0 ? K : 1][T];
+#pragma unroll
+            for (int j = 0; j < K; ++j)
+#pragma unroll
+                for (int t = 0; t < T; ++t) xl[j][t] = win[t * ld + li[j]];
+#pragma unroll
+            for (int j = 0; j < K; ++j)
+#pragma unroll
+                for (int t = 0; t < T; ++t) asm volatile("" : "+v"(xl[j][t]));
             V acc[T];
 #pragma unroll
             for (int t = 0; t < T; ++t) acc[t] = V(0);
 #pragma unroll
             for (int j = 0; j < K; ++j) {
 #pragma unroll
-                for (int t = 0; t < T; ++t)
-                    acc[t] += vj[j] * (win[t * ld + li[j]] + buffer_load<V>(rs[t], og[j]));
+                for (int t = 0; t < T; ++t) acc[t] += vj[j] * (nr[j] ? xl[j][t] : xg[j][t]);
             }
             if (live) {
 #pragma unroll
