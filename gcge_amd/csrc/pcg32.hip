@@ -451,6 +451,151 @@ __global__ __launch_bounds__(1024) void k_finish_rr(int nc, int mode, int nparts
 }
 
 // ---------------------------------------------------------------------
+// Three launches per iteration for systems of up to kFuseMaxRows rows.
+// There the fixed cost of a dependent launch outweighs the work, so the
+// two single-workgroup reductions fold into the kernels that follow them:
+// each workgroup of k_update_xr_pw forms alpha for its tile's columns from
+// the SpMM's partials (the arithmetic of k_finish_pw), and each workgroup
+// of k_update_p_rr forms rho and the stop flags of its tile's columns from
+// k_update_xr_pw's partials (the arithmetic of k_finish_rr), while its
+// workgroup (0, 0) writes the new state of every column and the compacted
+// tile list.  That state (rho, stop flags, tile list, count) is double
+// buffered by iteration parity, so no kernel overwrites what its own
+// workgroups still read; the sums are those of the five-launch iteration.
+// Past kFuseMaxRows each workgroup's share of the partials grows with the
+// row count and the separate reductions are cheaper.
+// ---------------------------------------------------------------------
+constexpr int64_t kFuseMaxRows = 65536;
+
+__global__ __launch_bounds__(kWG) void k_update_xr_pw(
+    int64_t n, const int *__restrict__ tiles, const int *__restrict__ ntact_p, int nparts_pw,
+    const float *__restrict__ partial_pw, const float *__restrict__ rho,
+    const int *__restrict__ active, const float *__restrict__ P, const float *__restrict__ Wv,
+    float *__restrict__ D, float *__restrict__ R, float *__restrict__ partial_rr)
+{
+    __shared__ float red[(kWG / 64 + 1) * kC];
+    __shared__ float s_alpha[kC];
+    const int k = blockIdx.y;
+    if (k >= *ntact_p) return;
+    const int tile = tiles[k];
+    // alpha of the tile's columns, one wavefront per column
+    const int wv = threadIdx.x >> 6, ln = threadIdx.x & 63;
+    for (int q = wv; q < kC; q += kWG / 64) {
+        const int c = tile * kC + q;
+        double sum = 0.0;
+        for (int i = ln; i < nparts_pw; i += 64) sum += partial_pw[(int64_t)c * nparts_pw + i];
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) sum += __shfl_xor(sum, o, 64);
+        if (ln == 0) s_alpha[q] = active[c] ? (float)((double)rho[c] / sum) : 0.f;
+    }
+    __syncthreads();
+    float a[kC], sq[kC];
+#pragma unroll
+    for (int t = 0; t < kC; ++t) {
+        a[t] = s_alpha[t];
+        sq[t] = 0.f;
+    }
+    const int64_t o = (int64_t)tile * n * kC;
+    for (int64_t i = (int64_t)blockIdx.x * kWG + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * kWG) {
+        const f8 p = load8(P + o + i * kC), w = load8(Wv + o + i * kC);
+        f8 d = load8(D + o + i * kC), r = load8(R + o + i * kC);
+#pragma unroll
+        for (int t = 0; t < kC; ++t) {
+            d.v[t] += a[t] * p.v[t];
+            r.v[t] -= a[t] * w.v[t];
+            sq[t] += r.v[t] * r.v[t];
+        }
+        store8(D + o + i * kC, d);
+        store8(R + o + i * kC, r);
+    }
+    block_sum8(sq, red);
+    if (threadIdx.x < kC)
+        partial_rr[(int64_t)(tile * kC + threadIdx.x) * gridDim.x + blockIdx.x] =
+            red[kWG / 64 * kC + threadIdx.x];
+}
+
+// rho and stop flag of column c from k_update_xr_pw's partials, as
+// k_finish_rr forms them; a stopped column keeps its state.
+__device__ __forceinline__ void next_state(int c, int nparts_rr,
+                                           const float *__restrict__ partial_rr,
+                                           const float *__restrict__ rho_o,
+                                           const int *__restrict__ active_o,
+                                           const float *__restrict__ init_res, float rate,
+                                           float tol, float &rho_n, int &act_n)
+{
+    rho_n = rho_o[c];
+    act_n = active_o[c];
+    if (act_n) {
+        double sum = 0.0;
+        for (int i = 0; i < nparts_rr; ++i) sum += partial_rr[(int64_t)c * nparts_rr + i];
+        const float res = (float)std::sqrt(sum);
+        rho_n = (float)sum;
+        act_n = (res > rate * init_res[c] && res > tol) ? 1 : 0;
+    }
+}
+
+__global__ __launch_bounds__(kWG) void k_update_p_rr(
+    int64_t n, int nc, int nparts_rr, const float *__restrict__ partial_rr,
+    const int *__restrict__ tiles_o, const int *__restrict__ ntact_o,
+    const float *__restrict__ rho_o, const int *__restrict__ active_o,
+    int *__restrict__ tiles_n, int *__restrict__ ntact_n, float *__restrict__ rho_n,
+    int *__restrict__ active_n, const float *__restrict__ init_res, float rate, float tol,
+    const float *__restrict__ R, float *__restrict__ P)
+{
+    __shared__ int tile_on[1024 / kC];
+    __shared__ float s_beta[kC];
+    __shared__ int s_on;
+    const int ntile = (nc + kC - 1) / kC;
+    if (blockIdx.x == 0 && blockIdx.y == 0) {
+        for (int t = threadIdx.x; t < ntile; t += kWG) tile_on[t] = 0;
+        __syncthreads();
+        for (int c = threadIdx.x; c < ntile * kC; c += kWG) {
+            float r;
+            int on;
+            next_state(c, nparts_rr, partial_rr, rho_o, active_o, init_res, rate, tol, r, on);
+            rho_n[c] = r;
+            active_n[c] = on;
+            if (on) tile_on[c / kC] = 1;  // benign race: all writers store 1
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            int q = 0;
+            for (int t = 0; t < ntile; ++t)
+                if (tile_on[t]) tiles_n[q++] = t;
+            *ntact_n = q;
+        }
+    }
+    const int k = blockIdx.y;
+    if (k >= *ntact_o) return;
+    const int tile = tiles_o[k];
+    if (threadIdx.x == 0) s_on = 0;
+    __syncthreads();
+    if (threadIdx.x < kC) {
+        const int c = tile * kC + threadIdx.x;
+        float r;
+        int on;
+        next_state(c, nparts_rr, partial_rr, rho_o, active_o, init_res, rate, tol, r, on);
+        s_beta[threadIdx.x] = rho_o[c] == 0.f ? 0.f : r / rho_o[c];
+        if (on) atomicOr(&s_on, 1);
+    }
+    __syncthreads();
+    if (!s_on) return;  // the tile has stopped: it leaves the list
+    float beta[kC];
+#pragma unroll
+    for (int t = 0; t < kC; ++t) beta[t] = s_beta[t];
+    const int64_t o = (int64_t)tile * n * kC;
+    for (int64_t i = (int64_t)blockIdx.x * kWG + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * kWG) {
+        const f8 r = load8(R + o + i * kC);
+        f8 p = load8(P + o + i * kC);
+#pragma unroll
+        for (int t = 0; t < kC; ++t) p.v[t] = r.v[t] + beta[t] * p.v[t];
+        store8(P + o + i * kC, p);
+    }
+}
+
+// ---------------------------------------------------------------------
 // Persistent BlockPCG for small systems: one cooperative launch runs every
 // inner iteration.  At a few thousand to tens of thousands of rows an
 // iteration moves a few MB, far less than the fixed cost of the five
@@ -741,10 +886,14 @@ struct PCG32 {
     float *rho = nullptr, *rho_old = nullptr, *init_res = nullptr, *alpha = nullptr;
     float *partial = nullptr;
     int *tiles = nullptr, *ntact = nullptr, *active = nullptr;
+    // three-launch iterations: the state of odd iterations, r.r partials
+    float *rho_b = nullptr, *partial_rr = nullptr;
+    int *tiles_b = nullptr, *ntact_b = nullptr, *active_b = nullptr;
     int *h_ntact = nullptr;
     int64_t iters = 0;
     // persistent loop: partial sums, barrier words and status
     bool use_persistent = false;
+    bool use_fused = true;  // three launches per iteration up to kFuseMaxRows rows
     float *ppart = nullptr;
     int *pctl = nullptr, *h_pstat = nullptr;
     int64_t persistent_calls = 0, persistent_fallbacks = 0;
@@ -759,12 +908,12 @@ struct PCG32 {
         int64_t n = 0;
         const void *a = nullptr, *as = nullptr, *av = nullptr, *b = nullptr, *bv = nullptr;
         float sigma = 0.f, rate = 0.f, tol = 0.f;
-        bool xc = false;
+        bool xc = false, fused = false;
         bool operator==(const GraphKey &o) const
         {
             return ntact == o.ntact && nc == o.nc && n == o.n && a == o.a && as == o.as &&
                    av == o.av && b == o.b && bv == o.bv && sigma == o.sigma && rate == o.rate &&
-                   tol == o.tol && xc == o.xc;
+                   tol == o.tol && xc == o.xc && fused == o.fused;
         }
     } gkey;
     void drop_graph()
@@ -793,6 +942,12 @@ struct PCG32 {
         HIPCHECK(hipMalloc(reinterpret_cast<void **>(&ntact), sizeof(int)));
         HIPCHECK(hipHostMalloc(reinterpret_cast<void **>(&h_ntact), sizeof(int),
                                hipHostMallocDefault));
+        HIPCHECK(hipMalloc(reinterpret_cast<void **>(&rho_b), sizeof(float) * ncap));
+        HIPCHECK(hipMalloc(reinterpret_cast<void **>(&active_b), sizeof(int) * ncap));
+        HIPCHECK(hipMalloc(reinterpret_cast<void **>(&tiles_b), sizeof(int) * (ncap / kC)));
+        HIPCHECK(hipMalloc(reinterpret_cast<void **>(&ntact_b), sizeof(int)));
+        HIPCHECK(hipMalloc(reinterpret_cast<void **>(&partial_rr),
+                           sizeof(float) * (size_t)ncap * 1024));
         HIPCHECK(hipMalloc(reinterpret_cast<void **>(&ppart),
                            sizeof(float) * 2 * kPersistMaxG * kPersistTiles * kC));
         HIPCHECK(hipMalloc(reinterpret_cast<void **>(&pctl), sizeof(int) * 4));
@@ -810,6 +965,11 @@ struct PCG32 {
         HIPCHECK(hipFree(active));
         HIPCHECK(hipFree(ntact));
         HIPCHECK(hipHostFree(h_ntact));
+        HIPCHECK(hipFree(rho_b));
+        HIPCHECK(hipFree(active_b));
+        HIPCHECK(hipFree(tiles_b));
+        HIPCHECK(hipFree(ntact_b));
+        HIPCHECK(hipFree(partial_rr));
         HIPCHECK(hipFree(ppart));
         HIPCHECK(hipFree(pctl));
         HIPCHECK(hipHostFree(h_pstat));
@@ -982,11 +1142,11 @@ void pcg32_run(void *matA, void **mv_b, void **mv_x, int *start, int *end, OPS *
         pcg32_run(matA, mv_b, mv_x, start, end, ops, false);
         return;
     }
-    auto spmm_pw = [&](bool x) {
+    auto spmm_pw = [&](bool x, const int *tl, const int *nt_p) {
 #define SPMM_PW(W, TWO, XC)                                                                   \
     k_spmm_pw<W, TWO, XC><<<nblk_row, kWG, 0, st>>>(n, A->sell.slice_ptr, A->sell.col,        \
-                                                    A->sell.val32, Bs->col, vB, sigma,         \
-                                                    S.tiles, S.ntact, S.p, S.w, S.partial)
+                                                    A->sell.val32, Bs->col, vB, sigma, tl,     \
+                                                    nt_p, S.p, S.w, S.partial)
         if (A->sell.max_width <= 8) {
             if (two) {
                 if (x) SPMM_PW(8, true, true); else SPMM_PW(8, true, false);
@@ -1006,33 +1166,59 @@ void pcg32_run(void *matA, void **mv_b, void **mv_x, int *start, int *end, OPS *
         // the tile list and count are set; p is written by the first
         // iteration before it is read, and w and the partials are
         // overwritten by every launch, so the timing runs leave no trace
-        const int best = tune_run(pkey, 2, 0, [&](int i) { spmm_pw(i == 1); }, st);
+        const int best =
+            tune_run(pkey, 2, 0, [&](int i) { spmm_pw(i == 1, S.tiles, S.ntact); }, st);
         xc = best == 1;
     }
-    // one inner iteration on st; `prof` records the kernel-class events
-    auto iteration = [&](int first, int nt, bool prof) {
+    // three launches per iteration up to kFuseMaxRows rows, with the state
+    // of iteration i in buffer i % 2 (the state the first finish wrote is
+    // buffer 0); five past it
+    const bool fused = S.use_fused && n <= kFuseMaxRows;
+    int *tl_buf[2] = {S.tiles, S.tiles_b}, *nt_buf[2] = {S.ntact, S.ntact_b};
+    int *act_buf[2] = {S.active, S.active_b};
+    float *rho_buf[2] = {S.rho, S.rho_b};
+    // one inner iteration on st; `prof` records the kernel-class events;
+    // `par` is the iteration's parity
+    auto iteration = [&](int first, int nt, bool prof, int par) {
         hipEvent_t evp = prof ? c.prof.begin(PROF_PCGVEC, (first ? 2.0 : 3.0) * tile_vec * nt,
                                              2.0 * kC * (double)n * nt, st)
                               : nullptr;
-        k_update_p<<<dim3(nb, nt), kWG, 0, st>>>(n, S.tiles, S.ntact, S.rho, S.rho_old, first,
-                                                 S.r, S.p);
+        if (!fused || first) {
+            k_update_p<<<dim3(nb, nt), kWG, 0, st>>>(n, S.tiles, S.ntact, S.rho, S.rho_old,
+                                                     first, S.r, S.p);
+        } else {
+            const int o = par ^ 1;
+            k_update_p_rr<<<dim3(nb, nt), kWG, 0, st>>>(
+                n, nc, (int)nb, S.partial_rr, tl_buf[o], nt_buf[o], rho_buf[o], act_buf[o],
+                tl_buf[par], nt_buf[par], rho_buf[par], act_buf[par], S.init_res,
+                (float)S.rate, (float)S.tol, S.r, S.p);
+        }
         if (prof) c.prof.end(evp, st);
+        const int *tl = fused ? tl_buf[par] : S.tiles;
+        const int *ntp = fused ? nt_buf[par] : S.ntact;
         hipEvent_t ev = prof ? c.prof.begin(PROF_SPMM32, mat_bytes + 2.0 * 4.0 * kC * (double)nt * n,
                                             2.0 * (double)A->sell.nnz_true * kC * nt, st)
                              : nullptr;
-        spmm_pw(xc);
+        spmm_pw(xc, tl, ntp);
         if (prof) c.prof.end(ev, st);
-        k_finish_pw<<<nt * kC, 64, 0, st>>>(S.tiles, S.ntact, (int)nblk_row, S.partial, S.rho,
-                                            S.active, S.alpha);
         hipEvent_t evx = prof ? c.prof.begin(PROF_PCGVEC, 6.0 * tile_vec * nt,
                                              6.0 * kC * (double)n * nt, st)
                               : nullptr;
-        k_update_xr<<<dim3(nb, nt), kWG, 0, st>>>(n, S.tiles, S.ntact, S.alpha, S.p, S.w, S.x,
-                                                  S.r, S.partial);
-        if (prof) c.prof.end(evx, st);
-        k_finish_rr<<<1, 1024, 0, st>>>(nc, 1, (int)nb, S.partial, S.tiles, S.ntact, S.active,
-                                        S.rho, S.rho_old, S.init_res, (float)S.rate,
-                                        (float)S.tol);
+        if (fused) {
+            k_update_xr_pw<<<dim3(nb, nt), kWG, 0, st>>>(
+                n, tl, ntp, (int)nblk_row, S.partial, rho_buf[par], act_buf[par], S.p, S.w, S.x,
+                S.r, S.partial_rr);
+            if (prof) c.prof.end(evx, st);
+        } else {
+            k_finish_pw<<<nt * kC, 64, 0, st>>>(S.tiles, S.ntact, (int)nblk_row, S.partial,
+                                                S.rho, S.active, S.alpha);
+            k_update_xr<<<dim3(nb, nt), kWG, 0, st>>>(n, S.tiles, S.ntact, S.alpha, S.p, S.w,
+                                                      S.x, S.r, S.partial);
+            if (prof) c.prof.end(evx, st);
+            k_finish_rr<<<1, 1024, 0, st>>>(nc, 1, (int)nb, S.partial, S.tiles, S.ntact,
+                                            S.active, S.rho, S.rho_old, S.init_res,
+                                            (float)S.rate, (float)S.tol);
+        }
     };
     while (it < S.max_iter && ntact > 0) {
         // the host reads the active-tile count every kCheck iterations; in
@@ -1057,6 +1243,7 @@ void pcg32_run(void *matA, void **mv_b, void **mv_x, int *start, int *end, OPS *
         key.rate = (float)S.rate;
         key.tol = (float)S.tol;
         key.xc = xc;
+        key.fused = fused;
         const bool have = S.gexec && key == S.gkey;
         std::unique_lock<std::mutex> solo;
         if (whole && S.use_graphs && !c.prof.on && !have) solo = sole_context_lock();
@@ -1065,7 +1252,8 @@ void pcg32_run(void *matA, void **mv_b, void **mv_x, int *start, int *end, OPS *
                 S.drop_graph();
                 hipGraph_t graph;
                 HIPCHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-                for (int k = 0; k < kCheck; ++k) iteration(0, ntact, false);
+                // groups start at multiples of kCheck (even): parities 0, 1, 0, 1
+                for (int k = 0; k < kCheck; ++k) iteration(0, ntact, false, k % 2);
                 HIPCHECK(hipStreamEndCapture(st, &graph));
                 HIPCHECK(hipGraphInstantiate(&S.gexec, graph, nullptr, nullptr, 0));
                 HIPCHECK(hipGraphDestroy(graph));
@@ -1075,11 +1263,15 @@ void pcg32_run(void *matA, void **mv_b, void **mv_x, int *start, int *end, OPS *
             HIPCHECK(hipGraphLaunch(S.gexec, st));
             it += kCheck;
         } else {
-            iteration(it == 0 ? 1 : 0, ntact, true);
+            iteration(it == 0 ? 1 : 0, ntact, true, it % 2);
             ++it;
         }
         if (it % kCheck == 0 || it == S.max_iter) {
-            HIPCHECK(hipMemcpyAsync(S.h_ntact, S.ntact, sizeof(int), hipMemcpyDeviceToHost, st));
+            // the latest state: with three launches, that of iteration it - 1
+            // (which counts the stops of iteration it - 2; a tile that stopped
+            // in the last iteration leaves the list in the next one)
+            const int *ntp = fused ? nt_buf[(it - 1) % 2] : S.ntact;
+            HIPCHECK(hipMemcpyAsync(S.h_ntact, ntp, sizeof(int), hipMemcpyDeviceToHost, st));
             c.sync();
             ntact = *S.h_ntact;
         }
@@ -1111,6 +1303,16 @@ extern "C" void HIPGCG_BlockPCG32_SetPersistent(OPS *ops, int on)
     if (ops->MultiLinearSolver != pcg32_solve || ops->multi_linear_solver_workspace == nullptr)
         fail("HIPGCG_BlockPCG32_SetPersistent: no device BlockPCG installed", __FILE__, __LINE__);
     static_cast<PCG32 *>(ops->multi_linear_solver_workspace)->use_persistent = on != 0;
+}
+
+extern "C" void HIPGCG_BlockPCG32_SetFused(OPS *ops, int on)
+{
+    using namespace gcge;
+    if (ops->MultiLinearSolver != pcg32_solve || ops->multi_linear_solver_workspace == nullptr)
+        fail("HIPGCG_BlockPCG32_SetFused: no device BlockPCG installed", __FILE__, __LINE__);
+    PCG32 *S = static_cast<PCG32 *>(ops->multi_linear_solver_workspace);
+    S->use_fused = on != 0;
+    S->drop_graph();
 }
 
 extern "C" void HIPGCG_BlockPCG32_Counts(OPS *ops, int64_t *persistent, int64_t *fallbacks)

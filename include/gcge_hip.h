@@ -297,6 +297,11 @@ void HIPGCG_BlockPCG32_SetGraphs(OPS *ops, int on);
  * again with one launch per step.  Counts: calls that ran in one launch,
  * calls that fell back.  No counterpart in the reference. */
 void HIPGCG_BlockPCG32_SetPersistent(OPS *ops, int on);
+/* Up to 65,536 rows an inner iteration is three launches, the two
+ * single-workgroup reductions being done by every workgroup of the kernel
+ * that follows each (default on; same sums, so the same bits as the
+ * five-launch iteration).  No counterpart in the reference. */
+void HIPGCG_BlockPCG32_SetFused(OPS *ops, int on);
 void HIPGCG_BlockPCG32_Counts(OPS *ops, int64_t *persistent, int64_t *fallbacks);
 
 /* ------------------------------------------------------------------ */

@@ -241,3 +241,47 @@ def test_persistent_loop_matches_launch_loop(hip, shifted, n_side):
     # different partial-sum grouping: the iterates agree to FP32 accuracy
     d = np.linalg.norm(out[1] - out[0]) / np.linalg.norm(out[0])
     assert d <= 1e-2, d
+
+
+@pytest.mark.parametrize("graphs", [0, 1])
+@pytest.mark.parametrize("rate", [0.0, 0.05])
+def test_three_launch_iteration_matches_five(setup, graphs, rate):
+    """Up to 65,536 rows the reductions run inside the following kernels
+    (state double-buffered by parity); the result is the five-launch
+    iteration's bit for bit, eager or from graphs, with tiles stopping part
+    way (rate 0.05) or never (rate 0)."""
+    table, lib, A, B, dA, dB = setup
+    n, nc = A.shape[0], 20
+    b = np.random.default_rng(46).standard_normal((n, nc))
+    hb = table.create_multivec(nc, dA)
+    hx = table.create_multivec(nc, dA)
+    table.upload(hb, b)
+    outs = []
+    try:
+        for fused in (1, 0, 1):
+            lib.HIPGCG_MultiLinearSolverSetup_BlockPCG32(81, rate, 0.0, None, 0.0, table.ops)
+            lib.HIPGCG_BlockPCG32_SetPersistent(table.ops, 0)
+            lib.HIPGCG_BlockPCG32_SetGraphs(table.ops, graphs)
+            lib.HIPGCG_BlockPCG32_SetFused(table.ops, fused)
+            table.upload(hx, np.zeros((n, nc)))
+            abi.slot(table.ops, "MultiLinearSolver")(dA.handle, hb, hx, ia(0, 0), ia(nc, nc),
+                                                     table.ops)
+            outs.append(table.download(hx, 0, nc, n))
+    finally:
+        lib.HIPGCG_BlockPCG32_SetFused(table.ops, 1)
+        lib.HIPGCG_BlockPCG32_SetGraphs(table.ops, 1)
+        table.destroy_multivec(hb, nc)
+        table.destroy_multivec(hx, nc)
+    for o in outs[1:]:
+        assert np.array_equal(o, outs[0])
+    r = np.linalg.norm(b - A @ outs[0], axis=0) / np.linalg.norm(b, axis=0)
+    assert r.max() < (1.0 if rate == 0.0 else rate * 1.01)
+
+
+def test_five_launch_iteration_above_threshold(hip):
+    """Past 65,536 rows the iteration keeps its separate reductions."""
+    A, B = P.stencil_pair(264, 2)  # 69,696 rows
+    table = HipTable()
+    dA = DeviceMatrix(A, fp32=True)
+    rel, _ = run(table, hip, A, dA, None, 0.0, None, None, 400, 1e-2, 9, persistent=False)
+    assert rel.max() <= 1.01e-2
