@@ -136,9 +136,14 @@ void orth_self(void **x, int start_x, int *end_x, void *B, const OrthParams &p, 
 // the B inner product: block classical Gram-Schmidt passes until the
 // largest coefficient is below reorth_tol, at most 1 + max_reorth
 // (src/ops_orth.c:231-269), or at most max_passes when that is positive.
+// first_norms, if given, receives for each column of the block the norm
+// of its first-pass coefficients: the size of its component along the
+// earlier (B-orthonormal) columns before their removal.
 void remove_previous(void **x, int start_x, int end_x, void *B, const OrthParams &p,
-                     void **mv_ws, OPS *ops, int max_passes = 0)
+                     void **mv_ws, OPS *ops, int max_passes = 0,
+                     std::vector<double> *first_norms = nullptr)
 {
+    if (first_norms) first_norms->clear();
     if (start_x <= 0 || end_x <= start_x) return;
     int s[2], e[2];
     double one = 1.0;
@@ -150,6 +155,14 @@ void remove_previous(void **x, int start_x, int end_x, void *B, const OrthParams
     if (max_passes > 0) passes = std::min(passes, max_passes);
     for (int it = 0; it < passes; ++it) {
         ops->MultiVecQtAP('S', 'N', x, B, x, 0, s, e, coef.data(), e[0] - s[0], mv_ws, ops);
+        if (it == 0 && first_norms) {
+            const int ld = e[0] - s[0];
+            for (int j = 0; j < e[1] - s[1]; ++j) {
+                double q = 0.0;
+                for (int i = 0; i < ld; ++i) q += coef[(size_t)ld * j + i] * coef[(size_t)ld * j + i];
+                first_norms->push_back(std::sqrt(q));
+            }
+        }
         double amax = 0.0;
         for (int64_t i = 0; i < len; ++i) {
             coef[i] = -coef[i];
@@ -205,10 +218,23 @@ void cholqr(void **x, int start_x, int *end_x, void *B, const OrthParams &p, voi
     // second pass leaves components of order eps * ||x||, Giraud, Langou
     // and Rozloznik 2005); the reference's absolute test on the largest
     // coefficient (50 eps) would run a third pass on unit-size columns
-    remove_previous(x, start_x, *end_x, B, p, mv_ws, ops, 2);
+    std::vector<double> along;
+    remove_previous(x, start_x, *end_x, B, p, mv_ws, ops, 2, &along);
     const int n = *end_x - start_x;
     std::vector<double> G((size_t)n * n), R, Rinv((size_t)n * n), dinv((size_t)n);
+    bool again = false;
     for (int pass = 0; pass < 2; ++pass) {
+        // A column that was nearly in the span of the earlier columns keeps
+        // a remainder of order eps times its old component along them;
+        // normalizing it (the first pass) scales that remainder up by the
+        // cancellation.  When some column lost more than four digits
+        // (component along the earlier columns over 1e4 times what is
+        // left), the earlier columns are removed once more before the
+        // second pass, as MGS reorthogonalizes a column whose norm dropped.
+        // Without it, a block meeting a converged degenerate cluster lost
+        // B-orthogonality and the Rayleigh-Ritz step returned spurious
+        // near-zero eigenvalues.
+        if (pass == 1 && again) remove_previous(x, start_x, *end_x, B, p, mv_ws, ops, 1);
         int s[2] = {start_x, start_x}, e[2] = {*end_x, *end_x};
         ops->MultiVecQtAP('S', 'S', x, B, x, 0, s, e, G.data(), n, mv_ws, ops);
         bool ok = true;
@@ -216,6 +242,8 @@ void cholqr(void **x, int start_x, int *end_x, void *B, const OrthParams &p, voi
             const double d = std::sqrt(std::max(G[(size_t)n * i + i], 0.0));
             ok = d >= p.zero_tol;
             dinv[i] = ok ? 1.0 / d : 0.0;
+            if (pass == 0 && ok && (size_t)i < along.size() && along[(size_t)i] > 1e4 * d)
+                again = true;
         }
         if (ok) {
             for (int j = 0; j < n; ++j)

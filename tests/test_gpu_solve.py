@@ -192,3 +192,20 @@ def test_eps_entry_point_mixed_and_diverged(hip):
     assert r.nconv == 10 and np.max(np.abs(r.eigr - exact) / exact) <= 1e-10
     r1 = S.eps_solve(prob.A, prob.B, 10, max_it=1)
     assert r1.reason == abi.GCGE_EPS_DIVERGED_ITS and r1.its == 1 and r1.nconv < 10
+
+
+def test_mixed_cholqr_degenerate_cluster(hip):
+    """3D 85^3, nev 32, block 6 (a request of the served stream): the 6-fold
+    eigenvalue cluster at position 27 makes new directions nearly dependent
+    on converged ones.  Block Cholesky QR normalizes what remains of them,
+    so the earlier columns are removed once more between its two passes;
+    without that the Rayleigh-Ritz step returned spurious near-zero
+    eigenvalues and the solve stopped at 26 pairs after 1000 iterations."""
+    prob = P.stencil_problem(85, 3, 32, name="3d85_nev32")
+    assert (prob.block_size, prob.nev_init, prob.nev_max) == (6, 18, 50)
+    r = S.solve_problem(prob, shift=0, mode="mixed", rand="device", seed=1072, max_iter=400,
+                        orth="cholqr")
+    k = r.nev_conv
+    assert k >= prob.nev
+    exact = P.analytic_eigenvalues(85, 3, k)
+    assert np.max(np.abs(r.eval[:k] - exact) / exact) <= 1e-10
