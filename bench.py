@@ -129,12 +129,13 @@ def run_path(prob, n, dim, mode, steps, warmup, mats, want_check=True):
             out["max_rel_eig_err_vs_analytic"] = float(np.max(np.abs(res.eval[:k] - exact) /
                                                               exact))
         X = res.evec[:, :k]
+        out["_evec"] = X  # kept for the principal angles between paths
         lam = res.eval[:k]
         R = prob.A @ X - (prob.B @ X) * lam[None, :]
         rr = np.sqrt(np.einsum("ij,ij->j", R, R)) / np.abs(lam)
         out["max_rel_residual"] = float(rr.max())
         out["eval_first_last"] = [float(lam[0]), float(lam[min(prob.nev, k) - 1])]
-        del R, X
+        del R
     return out
 
 
@@ -516,6 +517,18 @@ def main():
         a, b = (np.array(results[m]["eval"]) for m in ("fp64", "mixed"))
         k = min(len(a), len(b))
         line["max_rel_eig_diff_fp64_vs_mixed"] = float(np.max(np.abs(a[:k] - b[:k]) / a[:k]))
+        X1, X2 = results["fp64"].get("_evec"), results["mixed"].get("_evec")
+        if X1 is not None and X2 is not None:
+            from gcge_amd import checks
+            kc = checks.cluster_cut(a, min(X1.shape[1], X2.shape[1]))
+            line["subspace"] = {
+                "pairs": kc,
+                "max_sine_fp64_vs_mixed": checks.max_subspace_sine(X1[:, :kc], X2[:, :kc],
+                                                                   prob.B),
+                "note": "largest principal angle in the B inner product over the first "
+                        "pairs up to a cluster boundary"}
+    for r in results.values():
+        r.pop("_evec", None)
     if cpu is None:
         line["cpu_baseline_note"] = why
     print(json.dumps(line))
