@@ -100,7 +100,7 @@ class Request(C.Structure):
                 ("block_size", C.c_int), ("nev_init", C.c_int), ("nev_max", C.c_int),
                 ("max_iter", C.c_int), ("mode", C.c_int), ("shift", C.c_int),
                 ("tol_abs", C.c_double), ("tol_rel", C.c_double), ("gap_min", C.c_double),
-                ("seed", C.c_uint64)]
+                ("seed", C.c_uint64), ("x0", c_dbl_p), ("nev_given", C.c_int)]
 
 
 class Result(C.Structure):

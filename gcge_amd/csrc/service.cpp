@@ -68,6 +68,8 @@ int run_one(Job &j)
     gcge_result &r = *j.res;
     const bool mixed = q.mode == GCGE_MODE_MIXED;
     const int flags = mixed ? GCGE_HIP_MAT_FP32 : 0;
+    if (q.nev_given < 0 || q.nev_given > q.nev_init || (q.nev_given > 0 && q.x0 == nullptr))
+        return -1;
     const double t0 = now_s();
     void *A = HIPCCS_Create(q.A, flags);
     if (A == nullptr) return -1;
@@ -94,9 +96,12 @@ int run_one(Job &j)
     if (mixed) HIPGCG_MultiLinearSolverSetup_BlockPCG32(30, 1e-2, 1e-14, B, 0.0, ops);
     std::vector<double> eval((size_t)q.nev_max, 0.0);
     int nconv = q.nev;
+    // warm start: the given vectors in the first columns of evec, where
+    // the driver takes nevGiven initial vectors from
+    if (q.nev_given > 0) HIP_MultiVecSetFromHost(evec, 0, q.nev_given, q.x0, q.A->nrows);
     gcge_hip_synchronize();
     const double t1 = now_s();
-    ops->EigenSolver(A, B, eval.data(), evec, 0, &nconv, ops);
+    ops->EigenSolver(A, B, eval.data(), evec, q.nev_given, &nconv, ops);
     gcge_hip_synchronize();
     const double t2 = now_s();
     int64_t stats[4];

@@ -49,11 +49,19 @@ class Service:
 
     @staticmethod
     def request(prob, mode: str = "mixed", seed: int = 0, shift: int = 1,
-                max_iter: int = 1000):
+                max_iter: int = 1000, x0=None):
         """The gcge_request of a problems.CCSProblem (with the host arrays
-        it points to)."""
+        it points to).  x0: N x k initial vectors (k <= nev_init) for a warm
+        start, or None."""
         A, ka = _ccs(prob.A)
         keep = [A, ka]
+        px0, ngiven = None, 0
+        if x0 is not None:
+            x0 = np.asfortranarray(x0, dtype=np.float64)
+            if x0.shape[0] != prob.n or x0.shape[1] > prob.nev_init:
+                raise ValueError("x0 must be N x k with k <= nev_init")
+            keep.append(x0)
+            px0, ngiven = x0.ctypes.data_as(abi.c_dbl_p), x0.shape[1]
         pB = None
         if prob.B is not None:
             B, kb = _ccs(prob.B)
@@ -62,7 +70,7 @@ class Service:
         req = abi.Request(C.pointer(A), pB, prob.nev, prob.block_size, prob.nev_init,
                           prob.nev_max, max_iter,
                           abi.GCGE_MODE_MIXED if mode == "mixed" else abi.GCGE_MODE_FP64,
-                          shift, prob.tol_abs, prob.tol_rel, prob.gap_min, seed)
+                          shift, prob.tol_abs, prob.tol_rel, prob.gap_min, seed, px0, ngiven)
         return req, keep
 
     @staticmethod
@@ -72,8 +80,8 @@ class Service:
         return int(abi.load().gcge_request_footprint(C.byref(req)))
 
     def submit(self, prob, mode: str = "mixed", seed: int = 0, want_vectors: bool = False,
-               shift: int = 1) -> int:
-        req, keep = self.request(prob, mode, seed, shift)
+               shift: int = 1, x0=None) -> int:
+        req, keep = self.request(prob, mode, seed, shift, x0=x0)
         ev = np.zeros(prob.nev_max)
         vec = np.zeros((prob.n, prob.nev_max), order="F") if want_vectors else None
         res = abi.Result()

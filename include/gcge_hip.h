@@ -322,6 +322,12 @@ typedef struct {
     int shift;
     double tol_abs, tol_rel, gap_min;
     uint64_t seed; /* initial block from the device generator */
+    /* warm start: nev_given (<= nev_init) initial vectors, N x nev_given
+     * column-major with leading dimension N, valid until the request
+     * completes; they take the place of the first generated columns, as
+     * nevGiven does for the reference's EigenSolver (gcge.c:519). */
+    const double *x0;
+    int nev_given;
 } gcge_request;
 typedef struct {
     int status;     /* 1 pending, 0 done, -1 matrix rejected, -2 over budget */
@@ -331,7 +337,9 @@ typedef struct {
     double *eval;   /* caller's array of nev_max doubles, or NULL */
     double *evec;   /* caller's N x nev_max column-major array, or NULL */
 } gcge_result;
-/* Device bytes a request holds while it runs. */
+/* Device bytes a request holds while it runs.  A request with nev_given
+ * outside [0, nev_init], or nev_given > 0 without x0, is refused
+ * (status -1). */
 int64_t gcge_request_footprint(const gcge_request *req);
 void *gcge_service_create(int workers, int64_t hbm_budget_bytes);
 /* Queues a request; returns its ticket, or -1 (res->status says why).  The

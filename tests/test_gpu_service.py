@@ -73,3 +73,29 @@ def test_admission_by_footprint(hip):
             svc.submit(P.config2())       # can never fit this budget
     assert st["max_concurrent"] == 1 and all(r.concurrent == 1 for r in res)
     assert st["hbm_in_use"] == 0
+
+
+@pytest.mark.parametrize("mode", ["mixed", "fp64"])
+def test_warm_start_from_a_nearby_problem(hip, mode):
+    """A request warm-started (nevGiven) from the eigenvectors of a nearby
+    problem -- the same grid with coefficients perturbed by 1 % -- converges
+    in fewer iterations than cold, to the same eigenvalues and residual
+    bound as a dense solve."""
+    rng = np.random.default_rng([P.SEED_BASE, 2000])
+    n = 40
+    kappa = P.varcoef2d_coefficients(n, 0.5, rng)
+    A0, B = P.varcoef_pair_2d_from(n, kappa)
+    A1, _ = P.varcoef_pair_2d_from(n, kappa * (1.0 + 0.01 * rng.standard_normal(n * n)))
+    bs, ni, nm = P.slepc_block_policy(15)
+    p0 = P.CCSProblem("warm0", A0, B, 15, bs, ni, nm)
+    p1 = P.CCSProblem("warm1", A1, B, 15, bs, ni, nm)
+    with Service(workers=2) as svc:
+        base = svc.wait(svc.submit(p0, mode=mode, seed=21, want_vectors=True))
+        cold = svc.wait(svc.submit(p1, mode=mode, seed=21, want_vectors=True))
+        warm = svc.wait(svc.submit(p1, mode=mode, seed=21, want_vectors=True,
+                                   x0=base.evec[:, :ni]))
+        with pytest.raises(ValueError):
+            svc.submit(p1, x0=np.ones((p1.n, ni + 1)))   # more than nev_init
+    check(p1, cold)
+    check(p1, warm)
+    assert warm.num_iter < cold.num_iter, (warm.num_iter, cold.num_iter)
