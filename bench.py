@@ -201,12 +201,13 @@ KERNEL_OF_CLASS = {"lincomb": ["k_lincomb_mfma"], "spmm64": ["k_spmm_sell"],
                    "spmm32": ["k_spmm_pw"], "pcgvec": ["k_update_p", "k_update_xr"]}
 
 
-def _counter_means(path: str):
-    """{kernel name: (dispatches, mean)} from a --rocprof-summary output."""
+def _counter_means(path: str, counters=("FETCH_SIZE", "WRITE_SIZE")):
+    """{kernel name: (dispatches, mean)} from a --rocprof-summary output,
+    over the lines of the named counters."""
     out = {}
     for ln in open(path):
         f = ln.split(None, 3)
-        if len(f) == 4 and f[0].isdigit() and f[1] in ("FETCH_SIZE", "WRITE_SIZE"):
+        if len(f) == 4 and f[0].isdigit() and f[1] in counters:
             out[f[3].strip()] = (int(f[0]), float(f[2]))
     return out
 
@@ -267,6 +268,43 @@ def pmc_traffic(fetch: str, write: str, cfetch: str, cwrite: str, bench_json: st
         print(f"{mf[k][0]:10d} {mf[k][1] * 1024 * ff + mw[k][1] * 1024 * fw:12.4e}  {_short(k)}")
     for cls, meas in result.items():
         print(f"traffic_bytes_per_launch {cls} {meas:.6e}")
+
+
+# kernel class of the library profiler -> its MFMA kernel
+MFMA_CLASSES = {"gram": "k_gram_mfma", "lincomb": "k_lincomb_mfma"}
+
+
+def pmc_mfma(summary: str, bench_json: str):
+    """MFMA work per dispatch from SQ_INSTS_VALU_MFMA_MOPS_F64 (one count is
+    512 FP64 flops) and the busy fraction of the matrix cores (the derived
+    counter MfmaUtil), next to the algorithmic flops of the bench line."""
+    mops = _counter_means(summary, ("SQ_INSTS_VALU_MFMA_MOPS_F64",))
+    util = _counter_means(summary, ("MfmaUtil",))
+    busy = _counter_means(summary, ("SQ_VALU_MFMA_BUSY_CYCLES",))
+    line = json.loads(open(bench_json).read().strip().splitlines()[-1])
+    kern = line["paths"]["mixed"]["kernels"]
+    print("# per kernel class (mean over dispatches): MFMA flops counted, algorithmic flops "
+          "(2*rows*m*n), their ratio, MfmaUtil (%)")
+    print(f"{'class':8s} {'kernel':16s} {'dispatches':>10} {'MFMA flops':>12} "
+          f"{'algorithmic':>12} {'ratio':>6} {'MfmaUtil %':>10} {'TF/s (events)':>13}")
+    for cls, kname in MFMA_CLASSES.items():
+        if cls not in kern:
+            continue
+        m, u = _find(mops, kname), _find(util, kname)
+        if m is None:
+            continue
+        alg = kern[cls]["flops"] / kern[cls]["launches"]
+        uu = u[1] if u is not None else float("nan")
+        print(f"{cls:8s} {kname:16s} {m[0]:10d} {m[1] * 512:12.4e} {alg:12.4e} "
+              f"{m[1] * 512 / alg:6.3f} {uu:10.1f} {kern[cls]['TFs']:13.1f}")
+    print("# every kernel with MFMA work: dispatches, MFMA flops per dispatch, "
+          "MFMA busy cycles per dispatch, MfmaUtil (%)")
+    for k in sorted(mops, key=lambda k: -mops[k][1]):
+        if mops[k][1] <= 0:
+            continue
+        b = busy.get(k, (0, float("nan")))[1]
+        uu = util.get(k, (0, float("nan")))[1]
+        print(f"{mops[k][0]:10d} {mops[k][1] * 512:12.4e} {b:12.4e} {uu:6.1f}  {_short(k)}")
 
 
 def measured_traffic(kernel: str):
@@ -400,6 +438,7 @@ def main():
     ap.add_argument("--rocprof-summary", metavar="DIR")
     ap.add_argument("--pmc-traffic", nargs=5,
                     metavar=("FETCH", "WRITE", "COPY_FETCH", "COPY_WRITE", "BENCH_JSON"))
+    ap.add_argument("--pmc-mfma", nargs=2, metavar=("SUMMARY", "BENCH_JSON"))
     args = ap.parse_args()
     if args.tune_table is not None:
         # read by the library when its first device context is created
@@ -409,6 +448,9 @@ def main():
         return
     if args.pmc_traffic:
         pmc_traffic(*args.pmc_traffic)
+        return
+    if args.pmc_mfma:
+        pmc_mfma(*args.pmc_mfma)
         return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1",
