@@ -389,12 +389,30 @@ __global__ __launch_bounds__(64) void k_finish_pw(const int *__restrict__ tiles,
     const int k = blockIdx.x / kC, t = blockIdx.x % kC;
     if (k >= *ntact_p) return;
     const int c = tiles[k] * kC + t;
+    const float *pc = partial + (int64_t)c * nparts;
+    // lane l adds partials l, l + 64, l + 128, ... in that order; the loads
+    // of kBatch of them are issued before their adds, so the sum's latency
+    // is one load per batch rather than one per partial
+    constexpr int kBatch = 16;
     double s = 0.0;
-    for (int i = threadIdx.x; i < nparts; i += 64) s += partial[(int64_t)c * nparts + i];
+    int i = threadIdx.x;
+    for (; i + 64 * (kBatch - 1) < nparts; i += 64 * kBatch) {
+        float v[kBatch];
+#pragma unroll
+        for (int u = 0; u < kBatch; ++u) v[u] = pc[i + 64 * u];
+#pragma unroll
+        for (int u = 0; u < kBatch; ++u) s += v[u];
+    }
+    for (; i < nparts; i += 64) s += pc[i];
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
     if (threadIdx.x == 0) alpha[c] = active[c] ? (float)((double)rho[c] / s) : 0.f;
 }
+
+// k_finish_rr stages partials in LDS, kStageFloats at a time: rows of one
+// column's consecutive partials, the row pitch odd, so a wavefront reading
+// one element of 64 columns hits distinct banks
+constexpr int kStageFloats = 15360;
 
 // New residual norms, stop flags and the compacted tile list; one
 // workgroup, one thread per column (<= 1024).  mode 0: initial residual
@@ -411,15 +429,63 @@ __global__ __launch_bounds__(1024) void k_finish_rr(int nc, int mode, int nparts
                                                     float tol)
 {
     __shared__ int tile_on[1024 / kC];
+    __shared__ float stage[kStageFloats];
     const int c = threadIdx.x;
     const int ntile = (nc + kC - 1) / kC;
     if (c < ntile) tile_on[c] = 0;
     __syncthreads();
-    if (c < nc) {
-        const bool was = (mode == 0) ? true : active[c] != 0;
+    const bool was = c < nc && (mode == 0 || active[c] != 0);
+    // Each column's partials are summed in index order, as one thread
+    // walking them would.  The workgroup copies them into LDS in chunks of
+    // `chunk` per column (consecutive threads on consecutive partials of a
+    // column: coalesced), kBatch loads in flight per thread; nc * pitch
+    // stays within kStageFloats.
+    constexpr int kBatch = 8;
+    const int ncol = nc > 0 ? nc : 1;
+    int chunk = kStageFloats / ncol - 1;
+    if (chunk & 1) --chunk;  // pitch chunk + 1 odd
+    if (chunk > nparts) chunk = nparts;
+    if (chunk < 1) chunk = 1;
+    const int pitch = (chunk + 1) | 1;
+    double s = 0.0;
+    for (int i0 = 0; i0 < nparts; i0 += chunk) {
+        const int cnt = nparts - i0 < chunk ? nparts - i0 : chunk;
+        const int total = nc * cnt;
+        for (int e0 = 0; e0 < total; e0 += (int)blockDim.x * kBatch) {
+            float v[kBatch];
+            int dst[kBatch];
+            // every column's partials are read (those of stopped columns
+            // are stale and never summed); lanes past the end read the last
+            // element again and store nothing, so the loads carry no branch
+#pragma unroll
+            for (int u = 0; u < kBatch; ++u) {
+                const int e = e0 + u * (int)blockDim.x + threadIdx.x;
+                const int ec = e < total ? e : total - 1;
+                const int col = ec / cnt, j = ec - col * cnt;
+                v[u] = partial[(int64_t)col * nparts + i0 + j];
+                dst[u] = e < total ? col * pitch + j : -1;
+            }
+#pragma unroll
+            for (int u = 0; u < kBatch; ++u)
+                if (dst[u] >= 0) stage[dst[u]] = v[u];
+        }
+        __syncthreads();
         if (was) {
-            double s = 0.0;
-            for (int i = 0; i < nparts; ++i) s += partial[(int64_t)c * nparts + i];
+            const float *row = stage + c * pitch;
+            int j = 0;
+            for (; j + 8 <= cnt; j += 8) {
+                float w[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) w[u] = row[j + u];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) s += w[u];
+            }
+            for (; j < cnt; ++j) s += row[j];
+        }
+        __syncthreads();
+    }
+    if (c < nc) {
+        if (was) {
             const float res = (float)std::sqrt(s);
             int on;
             if (mode == 0) {
