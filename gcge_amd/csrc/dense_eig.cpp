@@ -158,13 +158,32 @@ int eigh_host(int n, double *A, int lda, double *w, double *Z, int ldz)
 
 namespace {
 // per thread (held by the thread's context): a rocBLAS handle on the
-// thread's stream and the device copies of the projected matrix
+// thread's stream, the device copies of the projected matrix and pinned
+// host staging for them (contiguous copies at DMA speed instead of 2D
+// copies from pageable memory)
 struct DevEig {
     rocblas_handle h = nullptr;
     double *dA = nullptr, *dW = nullptr, *dE = nullptr;
     rocblas_int *dinfo = nullptr;
+    double *hA = nullptr, *hW = nullptr;
+    rocblas_int *hinfo = nullptr;
     int cap = 0;
 };
+
+void free_buffers(DevEig &d)
+{
+    if (d.dA == nullptr) return;
+    HIPCHECK(hipFree(d.dA));
+    HIPCHECK(hipFree(d.dW));
+    HIPCHECK(hipFree(d.dE));
+    HIPCHECK(hipFree(d.dinfo));
+    HIPCHECK(hipHostFree(d.hA));
+    HIPCHECK(hipHostFree(d.hW));
+    HIPCHECK(hipHostFree(d.hinfo));
+    d.dA = d.dW = d.dE = d.hA = d.hW = nullptr;
+    d.dinfo = d.hinfo = nullptr;
+    d.cap = 0;
+}
 DevEig &deveig()
 {
     Ctx &c = ctx();
@@ -182,12 +201,7 @@ DevEig &deveig()
 void release_dev_eig(void *p)
 {
     DevEig *d = static_cast<DevEig *>(p);
-    if (d->dA) {
-        HIPCHECK(hipFree(d->dA));
-        HIPCHECK(hipFree(d->dW));
-        HIPCHECK(hipFree(d->dE));
-        HIPCHECK(hipFree(d->dinfo));
-    }
+    free_buffers(*d);
     rocblas_destroy_handle(d->h);
     delete d;
 }
@@ -199,30 +213,36 @@ int eigh_device(int n, double *A, int lda, double *w, double *Z, int ldz)
     Ctx &c = ctx();
     if (n > d.cap) {
         c.sync();
-        if (d.dA) {
-            HIPCHECK(hipFree(d.dA));
-            HIPCHECK(hipFree(d.dW));
-            HIPCHECK(hipFree(d.dE));
-            HIPCHECK(hipFree(d.dinfo));
-        }
+        free_buffers(d);
         d.cap = n + 64;
-        HIPCHECK(hipMalloc(reinterpret_cast<void **>(&d.dA), sizeof(double) * d.cap * d.cap));
+        const size_t sq = sizeof(double) * (size_t)d.cap * d.cap;
+        HIPCHECK(hipMalloc(reinterpret_cast<void **>(&d.dA), sq));
         HIPCHECK(hipMalloc(reinterpret_cast<void **>(&d.dW), sizeof(double) * d.cap));
         HIPCHECK(hipMalloc(reinterpret_cast<void **>(&d.dE), sizeof(double) * d.cap));
         HIPCHECK(hipMalloc(reinterpret_cast<void **>(&d.dinfo), sizeof(rocblas_int)));
+        HIPCHECK(hipHostMalloc(reinterpret_cast<void **>(&d.hA), sq, hipHostMallocDefault));
+        HIPCHECK(hipHostMalloc(reinterpret_cast<void **>(&d.hW), sizeof(double) * d.cap,
+                               hipHostMallocDefault));
+        HIPCHECK(hipHostMalloc(reinterpret_cast<void **>(&d.hinfo), sizeof(rocblas_int),
+                               hipHostMallocDefault));
     }
-    HIPCHECK(hipMemcpy2DAsync(d.dA, sizeof(double) * n, A, sizeof(double) * lda,
-                              sizeof(double) * n, n, hipMemcpyHostToDevice, c.stream));
+    const size_t col = sizeof(double) * (size_t)n;
+    for (int j = 0; j < n; ++j) memcpy(d.hA + (size_t)n * j, A + (size_t)lda * j, col);
+    HIPCHECK(hipMemcpyAsync(d.dA, d.hA, col * n, hipMemcpyHostToDevice, c.stream));
     rocblas_status st = rocsolver_dsyevd(d.h, rocblas_evect_original, rocblas_fill_upper, n,
                                          d.dA, n, d.dW, d.dE, d.dinfo);
-    if (st != rocblas_status_success) return -1000 - (int)st;
-    rocblas_int info = 0;
-    HIPCHECK(hipMemcpyAsync(&info, d.dinfo, sizeof(rocblas_int), hipMemcpyDeviceToHost, c.stream));
-    HIPCHECK(hipMemcpyAsync(w, d.dW, sizeof(double) * n, hipMemcpyDeviceToHost, c.stream));
-    HIPCHECK(hipMemcpy2DAsync(Z, sizeof(double) * ldz, d.dA, sizeof(double) * n,
-                              sizeof(double) * n, n, hipMemcpyDeviceToHost, c.stream));
+    if (st != rocblas_status_success) {
+        c.sync();  // the staging buffer stays in use until the copy is done
+        return -1000 - (int)st;
+    }
+    HIPCHECK(hipMemcpyAsync(d.hinfo, d.dinfo, sizeof(rocblas_int), hipMemcpyDeviceToHost,
+                            c.stream));
+    HIPCHECK(hipMemcpyAsync(d.hW, d.dW, col, hipMemcpyDeviceToHost, c.stream));
+    HIPCHECK(hipMemcpyAsync(d.hA, d.dA, col * n, hipMemcpyDeviceToHost, c.stream));
     c.sync();
-    return (int)info;
+    memcpy(w, d.hW, col);
+    for (int j = 0; j < n; ++j) memcpy(Z + (size_t)ldz * j, d.hA + (size_t)n * j, col);
+    return (int)*d.hinfo;
 }
 
 }  // namespace gcge
