@@ -19,11 +19,16 @@
 // row's 8 values are one contiguous 32-byte read per lane (2 KB per
 // wavefront).  Per iteration, over the tiles that still hold an active
 // column (their list is compacted on the device):
-//   k_update_p     p = r + beta p
+//   k_update_p     d += alpha' p (the previous iteration's step), p = r + beta p
 //   k_spmm_pw      w = (A + sigma B) p, partial p.w per workgroup
 //   k_finish_pw    alpha = rho / p.w (0 for stopped columns)
-//   k_update_xr    d += alpha p, r -= alpha w, partial r.r per workgroup
-//   k_finish_rr    rho, stop flags, compacted tile list
+//   k_update_xr    r -= alpha w, partial r.r per workgroup
+//   k_finish_rr    rho, stop flags, compacted tile list and the tiles it dropped
+// Deferring d += alpha p to the next k_update_p, which reads p anyway, saves
+// one pass over p per iteration (8 vector passes instead of 9); the tiles a
+// finish drops get their last step there too, and the iterations' last step
+// is applied after the loop.  Each d receives the same steps in the same
+// order as without the deferral.
 // and a 4-byte read of the active tile count by the host every kCheck
 // iterations.  Reductions are written per workgroup and summed in a fixed
 // order (no atomics).
@@ -305,46 +310,69 @@ __global__ __launch_bounds__(kWG) void k_spmm_pw(
 }
 
 // p = r + beta p on the listed tiles; p = r on the first iteration (p is
-// overwritten without being read: the workspace may hold anything)
-__global__ __launch_bounds__(kWG) void k_update_p(int64_t n, const int *__restrict__ tiles,
-                                                  const int *__restrict__ ntact_p,
-                                                  const float *__restrict__ rho,
-                                                  const float *__restrict__ rho_old, int first,
-                                                  const float *__restrict__ R,
-                                                  float *__restrict__ P)
+// overwritten without being read: the workspace may hold anything).
+// defer: also d += alpha p with the p being replaced and the previous
+// iteration's alpha, on the listed tiles and on the `dropped` tiles the last
+// finish took off the list (those get only the d step).  only_d: the d step
+// alone on both sets (the step of the last iteration, after the loop).
+__global__ __launch_bounds__(kWG) void k_update_p(
+    int64_t n, const int *__restrict__ tiles, const int *__restrict__ ntact_p,
+    const int *__restrict__ dropped, const int *__restrict__ ndropped_p,
+    const float *__restrict__ rho, const float *__restrict__ rho_old,
+    const float *__restrict__ alpha, int first, int defer, int only_d,
+    const float *__restrict__ R, float *__restrict__ P, float *__restrict__ D)
 {
     const int k = blockIdx.y;
-    if (k >= *ntact_p) return;
-    const int tile = tiles[k];
-    float beta[kC];
+    const int na = *ntact_p;
+    int tile;
+    bool step_p;
+    if (k < na) {
+        tile = tiles[k];
+        step_p = !only_d;
+    } else if (defer && !first && k < na + *ndropped_p) {
+        tile = dropped[k - na];
+        step_p = false;
+    } else {
+        return;
+    }
+    const bool step_d = defer && !first;
+    float beta[kC], a[kC];
 #pragma unroll
     for (int t = 0; t < kC; ++t) {
         const int c = tile * kC + t;
         beta[t] = (first || rho_old[c] == 0.f) ? 0.f : rho[c] / rho_old[c];
+        a[t] = step_d ? alpha[c] : 0.f;
     }
     const int64_t o = (int64_t)tile * n * kC;
     for (int64_t i = (int64_t)blockIdx.x * kWG + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * kWG) {
-        const f8 r = load8(R + o + i * kC);
-        f8 p;
         if (first) {
-            p = r;
-        } else {
-            p = load8(P + o + i * kC);
+            store8(P + o + i * kC, load8(R + o + i * kC));
+            continue;
+        }
+        f8 p = load8(P + o + i * kC);
+        if (step_d) {
+            f8 d = load8(D + o + i * kC);
+#pragma unroll
+            for (int t = 0; t < kC; ++t) d.v[t] += a[t] * p.v[t];
+            store8(D + o + i * kC, d);
+        }
+        if (step_p) {
+            const f8 r = load8(R + o + i * kC);
 #pragma unroll
             for (int t = 0; t < kC; ++t) p.v[t] = r.v[t] + beta[t] * p.v[t];
+            store8(P + o + i * kC, p);
         }
-        store8(P + o + i * kC, p);
     }
 }
 
-// d += alpha p, r -= alpha w on the listed tiles, with r.r partial sums
+// r -= alpha w on the listed tiles, with r.r partial sums (the d step of
+// this iteration is taken by the next k_update_p)
 __global__ __launch_bounds__(kWG) void k_update_xr(int64_t n, const int *__restrict__ tiles,
                                                    const int *__restrict__ ntact_p,
                                                    const float *__restrict__ alpha,
-                                                   const float *__restrict__ P,
                                                    const float *__restrict__ Wv,
-                                                   float *__restrict__ D, float *__restrict__ R,
+                                                   float *__restrict__ R,
                                                    float *__restrict__ partial)
 {
     __shared__ float red[(kWG / 64 + 1) * kC];
@@ -360,15 +388,13 @@ __global__ __launch_bounds__(kWG) void k_update_xr(int64_t n, const int *__restr
     const int64_t o = (int64_t)tile * n * kC;
     for (int64_t i = (int64_t)blockIdx.x * kWG + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * kWG) {
-        const f8 p = load8(P + o + i * kC), w = load8(Wv + o + i * kC);
-        f8 d = load8(D + o + i * kC), r = load8(R + o + i * kC);
+        const f8 w = load8(Wv + o + i * kC);
+        f8 r = load8(R + o + i * kC);
 #pragma unroll
         for (int t = 0; t < kC; ++t) {
-            d.v[t] += a[t] * p.v[t];
             r.v[t] -= a[t] * w.v[t];
             s[t] += r.v[t] * r.v[t];
         }
-        store8(D + o + i * kC, d);
         store8(R + o + i * kC, r);
     }
     block_sum8(s, red);
@@ -422,19 +448,23 @@ __global__ __launch_bounds__(1024) void k_finish_rr(int nc, int mode, int nparts
                                                     const float *__restrict__ partial,
                                                     int *__restrict__ tiles,
                                                     int *__restrict__ ntact_p,
+                                                    int *__restrict__ dropped,
+                                                    int *__restrict__ ndropped_p,
                                                     int *__restrict__ active,
                                                     float *__restrict__ rho,
                                                     float *__restrict__ rho_old,
                                                     float *__restrict__ init_res, float rate,
                                                     float tol)
 {
-    __shared__ int tile_on[1024 / kC];
+    __shared__ int tile_on[1024 / kC], tile_was[1024 / kC];
     __shared__ float stage[kStageFloats];
     const int c = threadIdx.x;
     const int ntile = (nc + kC - 1) / kC;
-    if (c < ntile) tile_on[c] = 0;
+    if (c < ntile) tile_on[c] = tile_was[c] = 0;
     __syncthreads();
     const bool was = c < nc && (mode == 0 || active[c] != 0);
+    // after an iteration the listed tiles are those with an active column
+    if (was && mode == 1) tile_was[c / kC] = 1;  // benign race: all writers store 1
     // Each column's partials are summed in index order, as one thread
     // walking them would.  The workgroup copies them into LDS in chunks of
     // `chunk` per column (consecutive threads on consecutive partials of a
@@ -509,10 +539,15 @@ __global__ __launch_bounds__(1024) void k_finish_rr(int nc, int mode, int nparts
     }
     __syncthreads();
     if (c == 0) {
-        int q = 0;
-        for (int t = 0; t < ntile; ++t)
-            if (tile_on[t]) tiles[q++] = t;
+        int q = 0, qd = 0;
+        for (int t = 0; t < ntile; ++t) {
+            if (tile_on[t])
+                tiles[q++] = t;
+            else if (tile_was[t])
+                dropped[qd++] = t;
+        }
         *ntact_p = q;
+        *ndropped_p = qd;
     }
 }
 
@@ -952,6 +987,7 @@ struct PCG32 {
     float *rho = nullptr, *rho_old = nullptr, *init_res = nullptr, *alpha = nullptr;
     float *partial = nullptr;
     int *tiles = nullptr, *ntact = nullptr, *active = nullptr;
+    int *dropped = nullptr, *ndropped = nullptr;  // tiles the last finish took off the list
     // three-launch iterations: the state of odd iterations, r.r partials
     float *rho_b = nullptr, *partial_rr = nullptr;
     int *tiles_b = nullptr, *ntact_b = nullptr, *active_b = nullptr;
@@ -968,16 +1004,18 @@ struct PCG32 {
     hipGraphExec_t gexec = nullptr;
     // everything the captured launches take by value: the launch grids
     // (active count, block width), the matrices, the shift, the stopping
-    // rule and the row-block order
+    // rule and the row-block order; the count before the latest read bounds
+    // the first k_update_p of a group
     struct GraphKey {
-        int ntact = -1, nc = 0;
+        int ntact = -1, ntact_prev = -1, nc = 0;
         int64_t n = 0;
         const void *a = nullptr, *as = nullptr, *av = nullptr, *b = nullptr, *bv = nullptr;
         float sigma = 0.f, rate = 0.f, tol = 0.f;
         bool xc = false, fused = false;
         bool operator==(const GraphKey &o) const
         {
-            return ntact == o.ntact && nc == o.nc && n == o.n && a == o.a && as == o.as &&
+            return ntact == o.ntact && ntact_prev == o.ntact_prev && nc == o.nc && n == o.n &&
+                   a == o.a && as == o.as &&
                    av == o.av && b == o.b && bv == o.bv && sigma == o.sigma && rate == o.rate &&
                    tol == o.tol && xc == o.xc && fused == o.fused;
         }
@@ -1006,6 +1044,8 @@ struct PCG32 {
         HIPCHECK(hipMalloc(reinterpret_cast<void **>(&tiles), sizeof(int) * (ncap / kC)));
         HIPCHECK(hipMalloc(reinterpret_cast<void **>(&active), sizeof(int) * ncap));
         HIPCHECK(hipMalloc(reinterpret_cast<void **>(&ntact), sizeof(int)));
+        HIPCHECK(hipMalloc(reinterpret_cast<void **>(&dropped), sizeof(int) * (ncap / kC)));
+        HIPCHECK(hipMalloc(reinterpret_cast<void **>(&ndropped), sizeof(int)));
         HIPCHECK(hipHostMalloc(reinterpret_cast<void **>(&h_ntact), sizeof(int),
                                hipHostMallocDefault));
         HIPCHECK(hipMalloc(reinterpret_cast<void **>(&rho_b), sizeof(float) * ncap));
@@ -1030,6 +1070,8 @@ struct PCG32 {
         HIPCHECK(hipFree(tiles));
         HIPCHECK(hipFree(active));
         HIPCHECK(hipFree(ntact));
+        HIPCHECK(hipFree(dropped));
+        HIPCHECK(hipFree(ndropped));
         HIPCHECK(hipHostFree(h_ntact));
         HIPCHECK(hipFree(rho_b));
         HIPCHECK(hipFree(active_b));
@@ -1127,8 +1169,9 @@ void pcg32_run(void *matA, void **mv_b, void **mv_x, int *start, int *end, OPS *
         }
 #undef RESID
     }
-    k_finish_rr<<<1, 1024, 0, st>>>(nc, 0, (int)nblk_row, S.partial, S.tiles, S.ntact, S.active,
-                                    S.rho, S.rho_old, S.init_res, (float)S.rate, (float)S.tol);
+    k_finish_rr<<<1, 1024, 0, st>>>(nc, 0, (int)nblk_row, S.partial, S.tiles, S.ntact,
+                                    S.dropped, S.ndropped, S.active, S.rho, S.rho_old,
+                                    S.init_res, (float)S.rate, (float)S.tol);
     HIPCHECK(hipMemcpyAsync(S.h_ntact, S.ntact, sizeof(int), hipMemcpyDeviceToHost, st));
     c.sync();
     int ntact = *S.h_ntact;
@@ -1140,9 +1183,11 @@ void pcg32_run(void *matA, void **mv_b, void **mv_x, int *start, int *end, OPS *
     int it = 0;
     // algorithmic bytes and flops of the vector updates over the active
     // tiles (as of the host's last read of the count, which can only
-    // overstate them: exact while no tile has stopped): p = r + beta p reads
-    // r (and p after the first iteration) and writes p; d += alpha p,
-    // r -= alpha w (with r.r) reads p, w, d, r and writes d, r
+    // overstate them: exact while no tile has stopped).  Five launches:
+    // d += alpha' p, p = r + beta p reads p, d, r and writes p, d (r read
+    // and p written on the first iteration); r -= alpha w (with r.r) reads
+    // w, r and writes r.  Three launches: p = r + beta p reads r, p and
+    // writes p; d += alpha p, r -= alpha w reads p, w, d, r and writes d, r.
     const double tile_vec = 4.0 * kC * (double)n;
     const float *vB = sigma != 0.f ? Bs->val32 : nullptr;
     // small systems, when asked for: the whole loop in one cooperative
@@ -1243,15 +1288,19 @@ void pcg32_run(void *matA, void **mv_b, void **mv_x, int *start, int *end, OPS *
     int *tl_buf[2] = {S.tiles, S.tiles_b}, *nt_buf[2] = {S.ntact, S.ntact_b};
     int *act_buf[2] = {S.active, S.active_b};
     float *rho_buf[2] = {S.rho, S.rho_b};
-    // one inner iteration on st; `prof` records the kernel-class events;
-    // `par` is the iteration's parity
-    auto iteration = [&](int first, int nt, bool prof, int par) {
-        hipEvent_t evp = prof ? c.prof.begin(PROF_PCGVEC, (first ? 2.0 : 3.0) * tile_vec * nt,
-                                             2.0 * kC * (double)n * nt, st)
+    // one inner iteration on st; `nt` bounds the tiles listed for it, `nt_p`
+    // those listed for the previous one (the five-launch k_update_p also
+    // takes the d step of the tiles the latest finish dropped); `prof`
+    // records the kernel-class events; `par` is the iteration's parity
+    auto iteration = [&](int first, int nt, int nt_p, bool prof, int par) {
+        const double pass_p = first ? 2.0 : (fused ? 3.0 : 5.0);
+        hipEvent_t evp = prof ? c.prof.begin(PROF_PCGVEC, pass_p * tile_vec * nt,
+                                             (fused ? 2.0 : 4.0) * kC * (double)n * nt, st)
                               : nullptr;
         if (!fused || first) {
-            k_update_p<<<dim3(nb, nt), kWG, 0, st>>>(n, S.tiles, S.ntact, S.rho, S.rho_old,
-                                                     first, S.r, S.p);
+            k_update_p<<<dim3(nb, fused ? nt : nt_p), kWG, 0, st>>>(
+                n, S.tiles, S.ntact, S.dropped, S.ndropped, S.rho, S.rho_old, S.alpha, first,
+                fused ? 0 : 1, 0, S.r, S.p, S.x);
         } else {
             const int o = par ^ 1;
             k_update_p_rr<<<dim3(nb, nt), kWG, 0, st>>>(
@@ -1267,8 +1316,8 @@ void pcg32_run(void *matA, void **mv_b, void **mv_x, int *start, int *end, OPS *
                              : nullptr;
         spmm_pw(xc, tl, ntp);
         if (prof) c.prof.end(ev, st);
-        hipEvent_t evx = prof ? c.prof.begin(PROF_PCGVEC, 6.0 * tile_vec * nt,
-                                             6.0 * kC * (double)n * nt, st)
+        hipEvent_t evx = prof ? c.prof.begin(PROF_PCGVEC, (fused ? 6.0 : 3.0) * tile_vec * nt,
+                                             (fused ? 6.0 : 4.0) * kC * (double)n * nt, st)
                               : nullptr;
         if (fused) {
             k_update_xr_pw<<<dim3(nb, nt), kWG, 0, st>>>(
@@ -1278,14 +1327,19 @@ void pcg32_run(void *matA, void **mv_b, void **mv_x, int *start, int *end, OPS *
         } else {
             k_finish_pw<<<nt * kC, 64, 0, st>>>(S.tiles, S.ntact, (int)nblk_row, S.partial,
                                                 S.rho, S.active, S.alpha);
-            k_update_xr<<<dim3(nb, nt), kWG, 0, st>>>(n, S.tiles, S.ntact, S.alpha, S.p, S.w,
-                                                      S.x, S.r, S.partial);
+            k_update_xr<<<dim3(nb, nt), kWG, 0, st>>>(n, S.tiles, S.ntact, S.alpha, S.w, S.r,
+                                                      S.partial);
             if (prof) c.prof.end(evx, st);
             k_finish_rr<<<1, 1024, 0, st>>>(nc, 1, (int)nb, S.partial, S.tiles, S.ntact,
-                                            S.active, S.rho, S.rho_old, S.init_res,
-                                            (float)S.rate, (float)S.tol);
+                                            S.dropped, S.ndropped, S.active, S.rho, S.rho_old,
+                                            S.init_res, (float)S.rate, (float)S.tol);
         }
     };
+    int last_nt = ntact;  // the tile count the latest iteration launched with
+    // the host's count before its latest read: an iteration right after a
+    // read takes the d step of tiles listed for the previous one, which the
+    // new count may not cover
+    int prev_nt = ntact;
     while (it < S.max_iter && ntact > 0) {
         // the host reads the active-tile count every kCheck iterations; in
         // between it launches with the last count it read, an upper bound
@@ -1310,6 +1364,7 @@ void pcg32_run(void *matA, void **mv_b, void **mv_x, int *start, int *end, OPS *
         key.tol = (float)S.tol;
         key.xc = xc;
         key.fused = fused;
+        key.ntact_prev = fused ? -1 : prev_nt;  // only five-launch groups use it
         const bool have = S.gexec && key == S.gkey;
         std::unique_lock<std::mutex> solo;
         if (whole && S.use_graphs && !c.prof.on && !have) solo = sole_context_lock();
@@ -1318,8 +1373,10 @@ void pcg32_run(void *matA, void **mv_b, void **mv_x, int *start, int *end, OPS *
                 S.drop_graph();
                 hipGraph_t graph;
                 HIPCHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-                // groups start at multiples of kCheck (even): parities 0, 1, 0, 1
-                for (int k = 0; k < kCheck; ++k) iteration(0, ntact, false, k % 2);
+                // groups start at multiples of kCheck (even): parities 0, 1,
+                // 0, 1; the first iteration follows a read of the count
+                for (int k = 0; k < kCheck; ++k)
+                    iteration(0, ntact, k == 0 ? prev_nt : ntact, false, k % 2);
                 HIPCHECK(hipStreamEndCapture(st, &graph));
                 HIPCHECK(hipGraphInstantiate(&S.gexec, graph, nullptr, nullptr, 0));
                 HIPCHECK(hipGraphDestroy(graph));
@@ -1329,9 +1386,10 @@ void pcg32_run(void *matA, void **mv_b, void **mv_x, int *start, int *end, OPS *
             HIPCHECK(hipGraphLaunch(S.gexec, st));
             it += kCheck;
         } else {
-            iteration(it == 0 ? 1 : 0, ntact, true, it % 2);
+            iteration(it == 0 ? 1 : 0, ntact, it % kCheck == 0 ? prev_nt : ntact, true, it % 2);
             ++it;
         }
+        last_nt = ntact;
         if (it % kCheck == 0 || it == S.max_iter) {
             // the latest state: with three launches, that of iteration it - 1
             // (which counts the stops of iteration it - 2; a tile that stopped
@@ -1339,10 +1397,22 @@ void pcg32_run(void *matA, void **mv_b, void **mv_x, int *start, int *end, OPS *
             const int *ntp = fused ? nt_buf[(it - 1) % 2] : S.ntact;
             HIPCHECK(hipMemcpyAsync(S.h_ntact, ntp, sizeof(int), hipMemcpyDeviceToHost, st));
             c.sync();
+            prev_nt = ntact;
             ntact = *S.h_ntact;
         }
     }
     S.iters += it;
+    if (!fused && it > 0) {
+        // the last iteration's d step, on the tiles it ran on: the list of
+        // its finish and the tiles that finish dropped (last_nt, the count
+        // its launches used, bounds both)
+        hipEvent_t evf = c.prof.begin(PROF_PCGVEC, 3.0 * tile_vec * last_nt,
+                                      2.0 * kC * (double)n * last_nt, st);
+        k_update_p<<<dim3(nb, last_nt), kWG, 0, st>>>(n, S.tiles, S.ntact, S.dropped,
+                                                      S.ndropped, S.rho, S.rho_old, S.alpha, 0,
+                                                      1, 1, S.r, S.p, S.x);
+        c.prof.end(evf, st);
+    }
     // x += d in FP64
     k_add_up<<<dim3(nblk_row, nc), kWG, 0, st>>>(n, nc, S.x, Xv->d + (int64_t)start[1] * Xv->ldd,
                                                  Xv->ldd);
