@@ -242,14 +242,14 @@ GramPlanM plan(int64_t nrows, int m, int n, bool lower_only)
 // and four C^T operands (from L1/L2: C is small) and issues 16 MFMAs, so
 // each operand feeds four matrix-core instructions.  Operands sit in a
 // ring of kRing one-step buffers: the loads for step s + kRing - 1 are
-// issued before the MFMAs of step s, and at 224 registers two waves share
+// issued before the MFMAs of step s, and at 230-246 registers two waves share
 // each SIMD, so HBM latency hides behind the other wave's MFMAs.  The
 // accumulator holds 16 consecutive rows of one column per lane group, so
 // the stores are contiguous.  Ct is C^T packed k-major: Ct[k * ldct + j].
 // A workgroup (4 waves) covers 256 rows; consecutive workgroups on one XCD
 // take the column tiles of the same rows.
 // ---------------------------------------------------------------------
-constexpr int kRing = 4;
+constexpr int kRing = 5;  // 4 and 5 tie without beta; with beta, 5 is 3-8 % faster
 
 template <int BM>
 __global__ __launch_bounds__(kWG) void k_lincomb_mfma(

@@ -196,14 +196,21 @@ __global__ __launch_bounds__(kWG) void k_resid64(
     }
 }
 
-// x[:, c] += d[c] in FP64, for c < nc
+// x[:, c] += d[c] in FP64, for c < nc: one thread per row of one tile
+// reads the row's 8 values of d at once (a workgroup per column would read
+// each 32-byte row segment of d eight times) and updates the tile's columns
+// of x, each store coalesced across the wavefront
 __global__ __launch_bounds__(kWG) void k_add_up(int64_t n, int nc, const float *__restrict__ D,
                                                 double *__restrict__ Xv, int64_t ldx)
 {
-    const int c = blockIdx.y;
-    if (c >= nc) return;
+    const int tile = blockIdx.y;
     const int64_t r = (int64_t)blockIdx.x * kWG + threadIdx.x;
-    if (r < n) Xv[r + (int64_t)c * ldx] += (double)D[((int64_t)(c / kC) * n + r) * kC + c % kC];
+    if (r >= n) return;
+    const f8 d = load8(D + ((int64_t)tile * n + r) * kC);
+    const int c0 = tile * kC;
+#pragma unroll
+    for (int t = 0; t < kC; ++t)
+        if (c0 + t < nc) Xv[r + (int64_t)(c0 + t) * ldx] += (double)d.v[t];
 }
 
 // zero the tiles of d
@@ -1239,7 +1246,7 @@ void pcg32_run(void *matA, void **mv_b, void **mv_x, int *start, int *end, OPS *
             if (S.h_pstat[0] == 0) {
                 S.persistent_calls += 1;
                 S.iters += S.h_pstat[1];
-                k_add_up<<<dim3(nblk_row, nc), kWG, 0, st>>>(
+                k_add_up<<<dim3(nblk_row, ntile), kWG, 0, st>>>(
                     n, nc, S.x, Xv->d + (int64_t)start[1] * Xv->ldd, Xv->ldd);
                 return;
             }
@@ -1414,8 +1421,8 @@ void pcg32_run(void *matA, void **mv_b, void **mv_x, int *start, int *end, OPS *
         c.prof.end(evf, st);
     }
     // x += d in FP64
-    k_add_up<<<dim3(nblk_row, nc), kWG, 0, st>>>(n, nc, S.x, Xv->d + (int64_t)start[1] * Xv->ldd,
-                                                 Xv->ldd);
+    k_add_up<<<dim3(nblk_row, ntile), kWG, 0, st>>>(n, nc, S.x,
+                                                    Xv->d + (int64_t)start[1] * Xv->ldd, Xv->ldd);
 }
 
 }  // namespace
