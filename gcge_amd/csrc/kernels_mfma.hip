@@ -242,16 +242,20 @@ GramPlanM plan(int64_t nrows, int m, int n, bool lower_only)
 // and four C^T operands (from L1/L2: C is small) and issues 16 MFMAs, so
 // each operand feeds four matrix-core instructions.  Operands sit in a
 // ring of kRing one-step buffers: the loads for step s + kRing - 1 are
-// issued before the MFMAs of step s, and at 230-246 registers two waves share
+// issued before the MFMAs of step s, and at 224-246 registers two waves share
 // each SIMD, so HBM latency hides behind the other wave's MFMAs.  The
 // accumulator holds 16 consecutive rows of one column per lane group, so
 // the stores are contiguous.  Ct is C^T packed k-major: Ct[k * ldct + j].
 // A workgroup (4 waves) covers 256 rows; consecutive workgroups on one XCD
 // take the column tiles of the same rows.
 // ---------------------------------------------------------------------
-constexpr int kRing = 5;  // 4 and 5 tie without beta; with beta, 5 is 3-8 % faster
+// Ring depth by row count: over 2 M rows a 5-step ring ties with 4 without
+// beta and is 2-8 % faster with it (m = 220-460); up to 65,536 rows, where
+// launches are latency-bound, 5 steps are 1-15 % slower, so smaller
+// problems keep 4.
+constexpr int64_t kRingDeepRows = int64_t(1) << 18;
 
-template <int BM>
+template <int BM, int kRing>
 __global__ __launch_bounds__(kWG) void k_lincomb_mfma(
     int64_t nrows, int m, int n, const double *__restrict__ X, int64_t ldx,
     const double *__restrict__ Ct, int ldct, double beta_scalar,
@@ -414,15 +418,23 @@ void lincomb_mfma(int64_t nrows, int m, int n, const double *X, int64_t ldx, con
     const int ntj = (n + 63) / 64;
     const int64_t nrt = (nrows + 255) / 256;
     const unsigned g = (unsigned)(nrt * ntj);
+    const bool deep = nrows >= kRingDeepRows;
+#define GCGE_LC_LAUNCH(BM)                                                                  \
+    do {                                                                                    \
+        if (deep)                                                                           \
+            k_lincomb_mfma<BM, 5><<<g, kWG, 0, s>>>(nrows, m, n, X, ldx, Ct, ldct,          \
+                                                    beta_scalar, beta, Y, ldy, ntj);        \
+        else                                                                                \
+            k_lincomb_mfma<BM, 4><<<g, kWG, 0, s>>>(nrows, m, n, X, ldx, Ct, ldct,          \
+                                                    beta_scalar, beta, Y, ldy, ntj);        \
+    } while (0)
     if (beta_mode == 0)
-        k_lincomb_mfma<0><<<g, kWG, 0, s>>>(nrows, m, n, X, ldx, Ct, ldct, beta_scalar, beta, Y,
-                                            ldy, ntj);
+        GCGE_LC_LAUNCH(0);
     else if (beta_mode == 1)
-        k_lincomb_mfma<1><<<g, kWG, 0, s>>>(nrows, m, n, X, ldx, Ct, ldct, beta_scalar, beta, Y,
-                                            ldy, ntj);
+        GCGE_LC_LAUNCH(1);
     else
-        k_lincomb_mfma<2><<<g, kWG, 0, s>>>(nrows, m, n, X, ldx, Ct, ldct, beta_scalar, beta, Y,
-                                            ldy, ntj);
+        GCGE_LC_LAUNCH(2);
+#undef GCGE_LC_LAUNCH
 }
 
 }  // namespace gcge
