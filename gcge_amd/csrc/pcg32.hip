@@ -24,14 +24,13 @@
 //   k_finish_pw    alpha = rho / p.w (0 for stopped columns)
 //   k_update_xr    r -= alpha w, partial r.r per workgroup
 //   k_finish_rr    rho, stop flags, compacted tile list and the tiles it dropped
-// Deferring d += alpha p to the next k_update_p, which reads p anyway, saves
-// one pass over p per iteration (8 vector passes instead of 9); the tiles a
-// finish drops get their last step there too, and the iterations' last step
-// is applied after the loop.  Each d receives the same steps in the same
-// order as without the deferral.
 // and a 4-byte read of the active tile count by the host every kCheck
 // iterations.  Reductions are written per workgroup and summed in a fixed
-// order (no atomics).
+// order (no atomics).  Deferring d += alpha p to the next k_update_p, which
+// reads p anyway, saves one pass over p per iteration (8 vector passes
+// instead of 9); the tiles a finish drops get their last step there too,
+// and the last iteration's step is applied after the loop.  Each d
+// receives the same steps in the same order as without the deferral.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
