@@ -285,3 +285,41 @@ def test_five_launch_iteration_above_threshold(hip):
     dA = DeviceMatrix(A, fp32=True)
     rel, _ = run(table, hip, A, dA, None, 0.0, None, None, 400, 1e-2, 9, persistent=False)
     assert rel.max() <= 1.01e-2
+
+
+@pytest.mark.parametrize("graphs", [0, 1])
+def test_block_columns_equal_columns_solved_alone(hip, graphs):
+    """Past 65,536 rows each column's steps depend on that column alone
+    (its alpha, beta, stop flag and partial sums), so a column of a block
+    solve must equal the same column solved by itself, bit for bit.  With
+    17 columns (three tiles) stopping at different iterations, this covers
+    the d step deferred into the next p update: a tile leaving the list,
+    at a read of the active count or between reads, takes its last step
+    once, and the loop's last step is applied after it."""
+    A, _ = P.stencil_pair(264, 2)  # 69,696 rows
+    n, nc = A.shape[0], 17
+    table = HipTable()
+    dA = DeviceMatrix(A, fp32=True)
+    b = np.random.default_rng(91).standard_normal((n, nc))
+    hb = table.create_multivec(nc, dA)
+    hx = table.create_multivec(nc, dA)
+    table.upload(hb, b)
+    try:
+        hip.HIPGCG_MultiLinearSolverSetup_BlockPCG32(400, 1e-2, 0.0, None, 0.0, table.ops)
+        hip.HIPGCG_BlockPCG32_SetPersistent(table.ops, 0)
+        hip.HIPGCG_BlockPCG32_SetGraphs(table.ops, graphs)
+        solve = abi.slot(table.ops, "MultiLinearSolver")
+        table.upload(hx, np.zeros((n, nc)))
+        solve(dA.handle, hb, hx, ia(0, 0), ia(nc, nc), table.ops)
+        block = table.download(hx, 0, nc, n)
+        for c in range(nc):
+            table.upload(hx, np.zeros((n, nc)))
+            solve(dA.handle, hb, hx, ia(c, c), ia(c + 1, c + 1), table.ops)
+            alone = table.download(hx, 0, nc, n)
+            assert np.array_equal(alone[:, c], block[:, c]), f"column {c}"
+            r = np.linalg.norm(b[:, c] - A @ alone[:, c]) / np.linalg.norm(b[:, c])
+            assert r <= 1.01e-2
+    finally:
+        hip.HIPGCG_BlockPCG32_SetGraphs(table.ops, 1)
+        table.destroy_multivec(hb, nc)
+        table.destroy_multivec(hx, nc)
