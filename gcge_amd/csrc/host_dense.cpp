@@ -7,7 +7,12 @@
 // This table provides those host operations over the LAPACKVEC layout
 // (gcge_dense_mv: column-major, ld >= nrows), with the slot semantics of
 // app/app_lapack.c:24-699.  Sizes here are at most a few hundred, so the
-// arithmetic is plain cache-ordered loops.
+// arithmetic is plain cache-ordered loops; the larger products spread their
+// output columns over a few threads (the GPU waits for them).
+#include <sched.h>
+
+#include <algorithm>
+#include <atomic>
 #include <cassert>
 #include <cmath>
 #include <cstdarg>
@@ -15,12 +20,53 @@
 #include <cstdlib>
 #include <cstring>
 #include <ctime>
+#include <thread>
 #include <vector>
 
 #include "gcge_ops.h"
 #include "gcge_hip.h"
 
 namespace {
+
+// Column-parallel loops: f(j) for j in [0, n), on up to kMaxThreads threads
+// (fewer when the process may run on fewer CPUs) once a product has
+// kParallelWork multiply-adds: the projected matrices of a 100-pair solve
+// (order 460 times 120 columns, 25 M) are split, those of small problems
+// (a few million, solved several at a time by the service's workers) are
+// not.  Each j is computed by one thread exactly as the serial loop
+// computes it, so the results do not depend on the thread count.
+constexpr int kMaxThreads = 8;
+constexpr double kParallelWork = 8e6;
+
+int usable_cpus()
+{
+    static const int n = [] {
+        cpu_set_t set;
+        CPU_ZERO(&set);
+        if (sched_getaffinity(0, sizeof(set), &set) != 0) return 1;
+        return std::max(1, CPU_COUNT(&set));
+    }();
+    return n;
+}
+
+template <class F>
+void for_columns(int n, double work, F &&f)
+{
+    int nt = work < kParallelWork ? 1 : std::min({kMaxThreads, usable_cpus(), n});
+    if (nt <= 1) {
+        for (int j = 0; j < n; ++j) f(j);
+        return;
+    }
+    std::atomic<int> next{0};
+    auto worker = [&]() {
+        for (int j; (j = next.fetch_add(1)) < n;) f(j);
+    };
+    std::vector<std::thread> pool;
+    pool.reserve((size_t)nt - 1);
+    for (int t = 1; t < nt; ++t) pool.emplace_back(worker);
+    worker();
+    for (auto &th : pool) th.join();
+}
 
 using mv = gcge_dense_mv;
 
@@ -56,7 +102,7 @@ void gemm_tn(int mc, int nc, int k, double alpha, const double *Q, int ldQ,
         double *c = C + (size_t)ldC * j + i;
         *c = (beta == 0.0) ? alpha * v : alpha * v + beta * *c;
     };
-    for (int j = 0; j < nc; ++j) {
+    for_columns(nc, (double)mc * nc * k, [&](int j) {
         const double *p = P + (size_t)ldP * j;
         int i = 0;
         for (; i + 4 <= mc; i += 4) {
@@ -81,14 +127,14 @@ void gemm_tn(int mc, int nc, int k, double alpha, const double *Q, int ldQ,
             put(i + 3, j, a3 + b3);
         }
         for (; i < mc; ++i) put(i, j, dot(Q + (size_t)ldQ * i, p, k));
-    }
+    });
 }
 
 // C (m x n, ldC) = A (m x k, ldA) * B (k x n, ldB), overwriting C.
 void gemm_nn(int m, int n, int k, const double *A, int ldA, const double *B, int ldB,
              double *C, int ldC)
 {
-    for (int j = 0; j < n; ++j) {
+    for_columns(n, (double)m * n * k, [&](int j) {
         double *c = C + (size_t)ldC * j;
         for (int i = 0; i < m; ++i) c[i] = 0.0;
         for (int p = 0; p < k; ++p) {
@@ -97,7 +143,7 @@ void gemm_nn(int m, int n, int k, const double *A, int ldA, const double *B, int
             const double *a = A + (size_t)ldA * p;
             for (int i = 0; i < m; ++i) c[i] += a[i] * b;
         }
-    }
+    });
 }
 
 }  // namespace
@@ -138,7 +184,9 @@ void gcge_host_DenseMatQtAP(char ntluA, char nsdC, int nrowsA, int ncolsA,
                 }
             }
         } else if (nsdC == 'S') {
-            for (int c = 0; c < ncolsC; ++c) {
+            // column c writes rows c.. of column c, then row c of the
+            // columns after it: no entry is written by two columns
+            for_columns(ncolsC, 0.5 * (double)ncolsC * ncolsC * k, [&](int c) {
                 const double *p = matP + (size_t)ldP * c;
                 for (int r = c; r < ncolsC; ++r) {
                     const double s = dot(matQ + (size_t)ldQ * r, p, k);
@@ -147,7 +195,7 @@ void gcge_host_DenseMatQtAP(char ntluA, char nsdC, int nrowsA, int ncolsA,
                 }
                 for (int r = c + 1; r < ncolsC; ++r)
                     matC[(size_t)ldC * r + c] = matC[(size_t)ldC * c + r];
-            }
+            });
         } else {
             gemm_tn(nrowsC, ncolsC, k, alpha, matQ, ldQ, matP, ldP, beta, matC, ldC);
         }
@@ -164,7 +212,7 @@ void gcge_host_DenseMatQtAP(char ntluA, char nsdC, int nrowsA, int ncolsA,
     if (ntluA == 'L' || ntluA == 'U') {
         // symmetric A, only the `ntluA` triangle is valid
         const int n = nrowsA;
-        for (int j = 0; j < ncolsC; ++j) {
+        for_columns(ncolsC, (double)n * n * ncolsC, [&](int j) {
             const double *p = matP + (size_t)ldP * j;
             double *w = W + (size_t)ldW * j;
             for (int i = 0; i < n; ++i) w[i] = 0.0;
@@ -181,7 +229,7 @@ void gcge_host_DenseMatQtAP(char ntluA, char nsdC, int nrowsA, int ncolsA,
                     w[c] += a[c] * pc + dot(a, p, c);
                 }
             }
-        }
+        });
     } else if (ntluA == 'T') {
         gemm_tn(nrowsA, ncolsC, ncolsA, 1.0, matA, ldA, matP, ldP, 0.0, W, ldW);
     } else {
