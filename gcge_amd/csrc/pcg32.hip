@@ -23,6 +23,7 @@
 //   k_spmm_pw      w = (A + sigma B) p, partial p.w per workgroup
 //   k_finish_pw    alpha = rho / p.w (0 for stopped columns)
 //   k_update_xr    r -= alpha w, partial r.r per workgroup
+//   k_sum_cols     the r.r sum of each active column (a workgroup per column)
 //   k_finish_rr    rho, stop flags, compacted tile list and the tiles it dropped
 // and a 4-byte read of the active tile count by the host every kCheck
 // iterations.  Reductions are written per workgroup and summed in a fixed
@@ -441,17 +442,50 @@ __global__ __launch_bounds__(64) void k_finish_pw(const int *__restrict__ tiles,
     if (threadIdx.x == 0) alpha[c] = active[c] ? (float)((double)rho[c] / s) : 0.f;
 }
 
-// k_finish_rr stages partials in LDS, kStageFloats at a time: rows of one
-// column's consecutive partials, the row pitch odd, so a wavefront reading
-// one element of 64 columns hits distinct banks
-constexpr int kStageFloats = 15360;
+// Column sums of the r.r partials, one workgroup per column: the workgroup
+// copies kSumChunk partials at a time into LDS (coalesced) and one thread
+// adds them in index order in FP64, the order a thread walking the column
+// would use, while the copies of the next chunk wait at the barrier.
+// mode 0: every column c < nc; mode 1: the active ones (the others keep
+// their last state and are not read).
+constexpr int kSumChunk = 8192;
 
-// New residual norms, stop flags and the compacted tile list; one
-// workgroup, one thread per column (<= 1024).  mode 0: initial residual
-// of every column c < nc; mode 1: after an iteration, over the active
-// columns.
-__global__ __launch_bounds__(1024) void k_finish_rr(int nc, int mode, int nparts,
-                                                    const float *__restrict__ partial,
+__global__ __launch_bounds__(256) void k_sum_cols(int nparts, int mode,
+                                                  const float *__restrict__ partial,
+                                                  const int *__restrict__ active,
+                                                  double *__restrict__ sums)
+{
+    __shared__ float stage[kSumChunk];
+    const int c = blockIdx.x;
+    if (mode == 1 && active[c] == 0) return;  // workgroup-uniform
+    const float *pc = partial + (int64_t)c * nparts;
+    double s = 0.0;
+    for (int i0 = 0; i0 < nparts; i0 += kSumChunk) {
+        const int cnt = nparts - i0 < kSumChunk ? nparts - i0 : kSumChunk;
+        for (int i = threadIdx.x; i < cnt; i += blockDim.x) stage[i] = pc[i0 + i];
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            int i = 0;
+            for (; i + 8 <= cnt; i += 8) {
+                float w[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) w[u] = stage[i + u];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) s += w[u];
+            }
+            for (; i < cnt; ++i) s += stage[i];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) sums[c] = s;
+}
+
+// New residual norms, stop flags and the compacted tile list from the
+// column sums of k_sum_cols; one workgroup, one thread per column
+// (<= 1024).  mode 0: initial residual of every column c < nc; mode 1:
+// after an iteration, over the active columns.
+__global__ __launch_bounds__(1024) void k_finish_rr(int nc, int mode,
+                                                    const double *__restrict__ sums,
                                                     int *__restrict__ tiles,
                                                     int *__restrict__ ntact_p,
                                                     int *__restrict__ dropped,
@@ -463,7 +497,6 @@ __global__ __launch_bounds__(1024) void k_finish_rr(int nc, int mode, int nparts
                                                     float tol)
 {
     __shared__ int tile_on[1024 / kC], tile_was[1024 / kC];
-    __shared__ float stage[kStageFloats];
     const int c = threadIdx.x;
     const int ntile = (nc + kC - 1) / kC;
     if (c < ntile) tile_on[c] = tile_was[c] = 0;
@@ -471,55 +504,7 @@ __global__ __launch_bounds__(1024) void k_finish_rr(int nc, int mode, int nparts
     const bool was = c < nc && (mode == 0 || active[c] != 0);
     // after an iteration the listed tiles are those with an active column
     if (was && mode == 1) tile_was[c / kC] = 1;  // benign race: all writers store 1
-    // Each column's partials are summed in index order, as one thread
-    // walking them would.  The workgroup copies them into LDS in chunks of
-    // `chunk` per column (consecutive threads on consecutive partials of a
-    // column: coalesced), kBatch loads in flight per thread; nc * pitch
-    // stays within kStageFloats.
-    constexpr int kBatch = 8;
-    const int ncol = nc > 0 ? nc : 1;
-    int chunk = kStageFloats / ncol - 1;
-    if (chunk & 1) --chunk;  // pitch chunk + 1 odd
-    if (chunk > nparts) chunk = nparts;
-    if (chunk < 1) chunk = 1;
-    const int pitch = (chunk + 1) | 1;
-    double s = 0.0;
-    for (int i0 = 0; i0 < nparts; i0 += chunk) {
-        const int cnt = nparts - i0 < chunk ? nparts - i0 : chunk;
-        const int total = nc * cnt;
-        for (int e0 = 0; e0 < total; e0 += (int)blockDim.x * kBatch) {
-            float v[kBatch];
-            int dst[kBatch];
-            // every column's partials are read (those of stopped columns
-            // are stale and never summed); lanes past the end read the last
-            // element again and store nothing, so the loads carry no branch
-#pragma unroll
-            for (int u = 0; u < kBatch; ++u) {
-                const int e = e0 + u * (int)blockDim.x + threadIdx.x;
-                const int ec = e < total ? e : total - 1;
-                const int col = ec / cnt, j = ec - col * cnt;
-                v[u] = partial[(int64_t)col * nparts + i0 + j];
-                dst[u] = e < total ? col * pitch + j : -1;
-            }
-#pragma unroll
-            for (int u = 0; u < kBatch; ++u)
-                if (dst[u] >= 0) stage[dst[u]] = v[u];
-        }
-        __syncthreads();
-        if (was) {
-            const float *row = stage + c * pitch;
-            int j = 0;
-            for (; j + 8 <= cnt; j += 8) {
-                float w[8];
-#pragma unroll
-                for (int u = 0; u < 8; ++u) w[u] = row[j + u];
-#pragma unroll
-                for (int u = 0; u < 8; ++u) s += w[u];
-            }
-            for (; j < cnt; ++j) s += row[j];
-        }
-        __syncthreads();
-    }
+    const double s = was ? sums[c] : 0.0;
     if (c < nc) {
         if (was) {
             const float res = (float)std::sqrt(s);
@@ -992,6 +977,7 @@ struct PCG32 {
     float *x = nullptr, *r = nullptr, *p = nullptr, *w = nullptr;
     float *rho = nullptr, *rho_old = nullptr, *init_res = nullptr, *alpha = nullptr;
     float *partial = nullptr;
+    double *colsum = nullptr;  // per-column sums of the r.r partials
     int *tiles = nullptr, *ntact = nullptr, *active = nullptr;
     int *dropped = nullptr, *ndropped = nullptr;  // tiles the last finish took off the list
     // three-launch iterations: the state of odd iterations, r.r partials
@@ -1047,6 +1033,7 @@ struct PCG32 {
             HIPCHECK(hipMalloc(reinterpret_cast<void **>(q), sizeof(float) * ncap));
         HIPCHECK(hipMalloc(reinterpret_cast<void **>(&partial),
                            sizeof(float) * (size_t)ncap * nparts_max()));
+        HIPCHECK(hipMalloc(reinterpret_cast<void **>(&colsum), sizeof(double) * ncap));
         HIPCHECK(hipMalloc(reinterpret_cast<void **>(&tiles), sizeof(int) * (ncap / kC)));
         HIPCHECK(hipMalloc(reinterpret_cast<void **>(&active), sizeof(int) * ncap));
         HIPCHECK(hipMalloc(reinterpret_cast<void **>(&ntact), sizeof(int)));
@@ -1073,6 +1060,7 @@ struct PCG32 {
         if (!x) return;
         ctx().sync();
         for (float *q : {x, r, p, w, rho, rho_old, init_res, alpha, partial}) HIPCHECK(hipFree(q));
+        HIPCHECK(hipFree(colsum));
         HIPCHECK(hipFree(tiles));
         HIPCHECK(hipFree(active));
         HIPCHECK(hipFree(ntact));
@@ -1175,9 +1163,10 @@ void pcg32_run(void *matA, void **mv_b, void **mv_x, int *start, int *end, OPS *
         }
 #undef RESID
     }
-    k_finish_rr<<<1, 1024, 0, st>>>(nc, 0, (int)nblk_row, S.partial, S.tiles, S.ntact,
-                                    S.dropped, S.ndropped, S.active, S.rho, S.rho_old,
-                                    S.init_res, (float)S.rate, (float)S.tol);
+    k_sum_cols<<<nc, 256, 0, st>>>((int)nblk_row, 0, S.partial, S.active, S.colsum);
+    k_finish_rr<<<1, 1024, 0, st>>>(nc, 0, S.colsum, S.tiles, S.ntact, S.dropped, S.ndropped,
+                                    S.active, S.rho, S.rho_old, S.init_res, (float)S.rate,
+                                    (float)S.tol);
     HIPCHECK(hipMemcpyAsync(S.h_ntact, S.ntact, sizeof(int), hipMemcpyDeviceToHost, st));
     c.sync();
     int ntact = *S.h_ntact;
@@ -1338,9 +1327,10 @@ void pcg32_run(void *matA, void **mv_b, void **mv_x, int *start, int *end, OPS *
             k_update_xr<<<dim3(nb, nt), kWG, 0, st>>>(n, S.tiles, S.ntact, S.alpha, S.w, S.r,
                                                       S.partial);
             if (prof) c.prof.end(evx, st);
-            k_finish_rr<<<1, 1024, 0, st>>>(nc, 1, (int)nb, S.partial, S.tiles, S.ntact,
-                                            S.dropped, S.ndropped, S.active, S.rho, S.rho_old,
-                                            S.init_res, (float)S.rate, (float)S.tol);
+            k_sum_cols<<<nc, 256, 0, st>>>((int)nb, 1, S.partial, S.active, S.colsum);
+            k_finish_rr<<<1, 1024, 0, st>>>(nc, 1, S.colsum, S.tiles, S.ntact, S.dropped,
+                                            S.ndropped, S.active, S.rho, S.rho_old, S.init_res,
+                                            (float)S.rate, (float)S.tol);
         }
     };
     int last_nt = ntact;  // the tile count the latest iteration launched with
