@@ -307,11 +307,12 @@ def pmc_mfma(summary: str, bench_json: str):
         print(f"{mops[k][0]:10d} {mops[k][1] * 512:12.4e} {b:12.4e} {uu:6.1f}  {_short(k)}")
 
 
-def measured_traffic(kernel: str):
-    """HBM bytes per launch of `kernel` from the committed counter summary:
-    a line 'traffic_bytes_per_launch <kernel> <bytes>'."""
+def measured_traffic(kernel: str, config: str):
+    """HBM bytes per launch of `kernel` from the committed counter summary
+    of the 128^3 mixed solve (a line 'traffic_bytes_per_launch <kernel>
+    <bytes>'); None for other workloads, whose traffic was not measured."""
     p = os.path.join(ROOT, "profiles", "pmc_traffic_cfg2_mixed.txt")
-    if not os.path.exists(p):
+    if config != "cfg2" or not os.path.exists(p):
         return None
     for ln in open(p):
         f = ln.split()
@@ -516,7 +517,7 @@ def main():
                   "algorithmic_bytes_per_launch": kd["bytes"] / kd["launches"],
                   "algorithmic_flops_per_launch": kd["flops"] / kd["launches"],
                   "flop_per_byte": intensity,
-                  "traffic": measured_traffic(dom)}
+                  "traffic": measured_traffic(dom, args.config)}
         if intensity > FP64_PEAK_TFS / (HBM_PEAK_GBS / 1e3):
             roof = {"bound": "mfma", "achieved": kd["TFs"], "peak": FP64_PEAK_TFS,
                     "unit": "TFLOP/s", "frac": kd["TFs"] / FP64_PEAK_TFS, **common}
