@@ -231,8 +231,11 @@ __global__ __launch_bounds__(kWG) void k_zero_tiles(int64_t n, float *__restrict
 // partial[(tile*8 + t) * nblk + block] = the block's sum of p * w.
 // TWO: B is a matrix of its own (a shift by a B other than A); otherwise
 // the operator is A (+ sigma A) and only A's entries are held in registers.
+// At most 128 VGPRs (four waves per SIMD) except with B's entries gathered
+// separately, where the entry offsets and values of both matrices do not
+// fit: the unbounded 32-entry kernel took 130 and ran three waves.
 template <int W, bool TWO, bool XC>
-__global__ __launch_bounds__(kWG) void k_spmm_pw(
+__global__ __launch_bounds__(kWG, (W == 32 && TWO) ? 1 : 4) void k_spmm_pw(
     int64_t n, const int64_t *__restrict__ slice_ptr, const int *__restrict__ colA,
     const float *__restrict__ valA, const int *__restrict__ colB,
     const float *__restrict__ valB, float sigma, const int *__restrict__ tiles,
