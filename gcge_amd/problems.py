@@ -1,7 +1,7 @@
 """Synthetic generalized eigenproblems A x = lambda B x, stored CCS.
 
 The constant-coefficient stencil problems have closed-form spectra; they
-are the analytic oracles of the eigensolver.  All matrices are symmetric,
+give the eigensolver's results an exact answer to meet.  All matrices are symmetric,
 so their CCS arrays are also their CSR arrays (the HIP backend reads them
 as a row gather).
 
