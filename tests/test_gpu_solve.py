@@ -41,6 +41,12 @@ def check_pairs(prob, res, dim, n, iter_ref=None, iter_slack=6, nconv_ref=None):
     if res.evec is not None:
         rr = O.relative_residuals(prob.A, prob.B, lam, res.evec[:, :k])
         assert rr.max() <= 1e-8 * 1.01, rr.max()
+        # B-orthonormal eigenvectors (the B inner product of src/ops_orth.c);
+        # a block that lost B-orthogonality shows here before it shows as a
+        # spurious eigenvalue
+        X = res.evec[:, :k]
+        G = X.T @ (prob.B @ X)
+        assert np.abs(G - np.eye(k)).max() <= 1e-10
     if iter_ref is not None:
         assert abs(res.num_iter - iter_ref) <= iter_slack, (res.num_iter, iter_ref)
     if nconv_ref is not None:

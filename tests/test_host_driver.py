@@ -24,6 +24,13 @@ def golden(name, shift, app):
     raise KeyError((name, shift, app))
 
 
+def b_orth_defect(B, X):
+    """max |X^T B X - I|: the eigenvectors leave the driver B-orthonormal
+    (the B inner product of src/ops_orth.c, kept through Rayleigh-Ritz)."""
+    G = X.T @ (B @ X)
+    return np.abs(G - np.eye(G.shape[0])).max()
+
+
 @pytest.mark.parametrize("name,n,dim,nev,shift", [("2d_32", 32, 2, 10, 1),
                                                   ("2d_32", 32, 2, 10, 0),
                                                   ("2d_24", 24, 2, 20, 1)])
@@ -39,6 +46,7 @@ def test_host_driver_matches_reference(hip, name, n, dim, nev, shift):
     assert np.max(np.abs(r.eval[:k] - ref) / ref) <= 1e-9
     rr = O.relative_residuals(prob.A, prob.B, r.eval[:k], r.evec[:, :k])
     assert rr.max() <= 1.01e-8
+    assert b_orth_defect(prob.B, r.evec[:, :r.nev_conv]) <= 1e-12
 
 
 @pytest.mark.parametrize("name,n,dim,nev,shift", [("2d_32", 32, 2, 10, 1),
@@ -61,6 +69,7 @@ def test_host_driver_cholqr(hip, name, n, dim, nev, shift):
     assert np.max(np.abs(r.eval[:k] - ref) / ref) <= 1e-9
     rr = O.relative_residuals(prob.A, prob.B, r.eval[:k], r.evec[:, :k])
     assert rr.max() <= 1.01e-8
+    assert b_orth_defect(prob.B, r.evec[:, :r.nev_conv]) <= 1e-12
 
 
 def test_host_driver_max_iter_reports_nonconvergence(hip):
