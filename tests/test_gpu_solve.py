@@ -210,11 +210,14 @@ def test_mixed_cholqr_degenerate_cluster(hip):
     prob = P.stencil_problem(85, 3, 32, name="3d85_nev32")
     assert (prob.block_size, prob.nev_init, prob.nev_max) == (6, 18, 50)
     r = S.solve_problem(prob, shift=0, mode="mixed", rand="device", seed=1072, max_iter=400,
-                        orth="cholqr")
+                        orth="cholqr", want_vectors=True)
     k = r.nev_conv
     assert k >= prob.nev
     exact = P.analytic_eigenvalues(85, 3, k)
     assert np.max(np.abs(r.eval[:k] - exact) / exact) <= 1e-10
+    X = r.evec[:, :k]
+    G = X.T @ (prob.B @ X)
+    assert np.abs(G - np.eye(k)).max() <= 1e-10
 
 
 @pytest.mark.parametrize("mode", ["fp64", "mixed"])
