@@ -142,9 +142,7 @@ EXPORTS = {
     "HIPCCS_Create": (C.c_void_p, [C.POINTER(CCSMat), C.c_int]),
     "HIPCCS_Destroy": (None, [C.POINTER(C.c_void_p)]),
     "HIPCCS_CreateGrid": (C.c_void_p, [C.POINTER(GridOp), C.c_int]),
-    "HIPGCG_BlockPCG32_SetPersistent": (None, [OPS_p, C.c_int]),
     "HIPGCG_BlockPCG32_SetFused": (None, [OPS_p, C.c_int]),
-    "HIPGCG_BlockPCG32_Counts": (None, [OPS_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
     "gcge_tune_set_mode": (None, [C.c_int]),
     "gcge_tune_get_mode": (C.c_int, []),
     "gcge_tune_load": (C.c_int, [C.c_char_p]),

@@ -43,6 +43,13 @@ void fail(const char *what, const char *file, int line)
     abort();
 }
 
+void record_error(const char *what)
+{
+    std::lock_guard<std::mutex> lk(g_err_mu);
+    g_err = std::string("gcge_hip: ") + what;
+    g_err_code = 1;
+}
+
 void hip_check(hipError_t e, const char *expr, const char *file, int line)
 {
     if (e == hipSuccess) return;
@@ -502,11 +509,12 @@ void inner_prod_impl(char nsd, MV *X, MV *Y, int *start, int *end, double *ip, i
     }
     const bool sym = (nsd == 'S');
     if (sym && nr != nc) fail("MultiVecInnerProd 'S': block not square", __FILE__, __LINE__);
-    double *part = c.scratch(gram_scratch(X->nrows, nr, nc, sym));
+    const GramPlan plan = gram_plan(X->nrows, nr, nc, sym);
+    double *part = c.scratch(plan.scratch);
     double *d = c.small((int64_t)nr * nc);
     hipEvent_t ev = c.prof.begin(PROF_GRAM, 8.0 * (double)X->nrows * (nr + nc),
                                  2.0 * (double)X->nrows * nr * nc, c.stream);
-    gram(X->nrows, nr, nc, xs, X->ldd, ys, Y->ldd, d, nr, sym, part, c.stream);
+    gram(plan, X->nrows, nr, nc, xs, X->ldd, ys, Y->ldd, d, nr, sym, part, c.stream);
     c.prof.end(ev, c.stream);
     double *h = c.pinned((int64_t)nr * nc);
     HIPCHECK(hipMemcpyAsync(h, d, sizeof(double) * nr * nc, hipMemcpyDeviceToHost, c.stream));

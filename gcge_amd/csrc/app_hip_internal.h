@@ -12,6 +12,9 @@
 namespace gcge {
 
 [[noreturn]] void fail(const char *what, const char *file, int line);
+// Records a failure for gcge_hip_last_error without aborting (entry points
+// that report errors by return value).
+void record_error(const char *what);
 void hip_check(hipError_t e, const char *expr, const char *file, int line);
 #define HIPCHECK(x) ::gcge::hip_check((x), #x, __FILE__, __LINE__)
 

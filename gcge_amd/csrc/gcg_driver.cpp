@@ -7,6 +7,14 @@
 // the reference.  State lives in one heap object (ops->
 // eigen_solver_workspace) instead of file-scope statics, so several
 // tables can solve independently.
+//
+// Licence note: the phases, their step order, the parameter defaults and
+// the monitor strings below follow the reference GCGE (xie-group/gcge,
+// src/ops_eig_sol_gcg.c, src/ops_lin_sol.c, src/ops_orth.c; Yu Li et al.),
+// which is distributed under the GNU Lesser General Public License 2.1.
+// This file is a close restatement of that algorithm in new C++ (the FP64
+// path must reproduce the reference's trajectory), so it is a work based
+// on the reference and is distributed under the same LGPL-2.1 terms.
 #include <algorithm>
 #include <cfloat>
 #include <cmath>
@@ -18,6 +26,12 @@
 
 #include "dense_eig.h"
 #include "gcge_hip.h"
+
+namespace gcge {
+// Inner iterations the device FP32 BlockPCG of `ops` has run since it was
+// installed (0 when the slot holds another solver); pcg32.hip.
+int64_t pcg32_iterations(OPS *ops);
+}
 
 namespace {
 
@@ -63,6 +77,7 @@ struct Solver {
     // statistics
     int numIter = 0;
     int64_t bpcg_iters = 0, linsolves = 0;
+    int64_t pcg32_base = 0;  // the device BlockPCG's count when the solve began
     double times[10] = {0};
 };
 
@@ -764,6 +779,7 @@ void gcg(void *A, void *B, double *eval, void **evec, int nevGiven, int *nevConv
     S.B = B;
     for (double &t : S.times) t = 0.0;
     S.bpcg_iters = 0;
+    S.pcg32_base = gcge::pcg32_iterations(ops);
     S.cholqr_blocks = 0;
     S.cholqr_fallbacks = 0;
     S.linsolves = 0;
@@ -986,7 +1002,9 @@ void HIPGCG_GetStats(OPS *ops, int64_t *stats, double *times)
     if (stats) {
         stats[0] = S.numIter;
         stats[1] = S.sizeC;
-        stats[2] = S.bpcg_iters;
+        // FP64 BlockPCG iterations of this driver plus those of a device
+        // FP32 BlockPCG installed in the MultiLinearSolver slot
+        stats[2] = S.bpcg_iters + (gcge::pcg32_iterations(ops) - S.pcg32_base);
         stats[3] = S.linsolves;
     }
     if (times)

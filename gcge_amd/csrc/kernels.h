@@ -50,11 +50,24 @@ int64_t dots_scratch(int ncols);
 void col_dots(int64_t nrows, int ncols, const double *X, int64_t ldx, const double *Y,
               int64_t ldy, double *out, double *partial, hipStream_t s);
 // G (m x n, ldg) = X^T Y with X: nrows x m, Y: nrows x n.  lower_only
-// computes only entries with row >= col (the 'S' case).  partial is scratch
-// of gram_scratch(nrows, m, n, lower_only) doubles.
-int64_t gram_scratch(int64_t nrows, int m, int n, bool lower_only);
-void gram(int64_t nrows, int m, int n, const double *X, int64_t ldx, const double *Y,
-          int64_t ldy, double *G, int ldg, bool lower_only, double *partial,
+// computes only entries with row >= col (the 'S' case).  The plan fixes
+// the kernel, its row chunking and the doubles of partial-sum scratch it
+// needs; it is computed once per call and used both to size the scratch
+// and to launch, so a change of the tuner table by another thread between
+// the two cannot make the launch need more scratch than was sized.
+struct GramPlan {
+    bool thin = false;    // streaming kernel (a side of at most 8 columns)
+    int choice = -1;      // matrix-core kernel: row-chunking candidate
+    bool tune = false;    // time every chunking first, on a private buffer
+    int64_t scratch = 0;  // doubles of partial sums (0 or less: nothing fits)
+};
+GramPlan gram_plan(int64_t nrows, int m, int n, bool lower_only);
+// A plan whose partial sums fit in max_scratch doubles and that never
+// tunes (the table's choice when it fits, else the one with the fewest row
+// chunks that does); scratch <= 0 when no chunking fits.
+GramPlan gram_plan_bounded(int64_t nrows, int m, int n, bool lower_only, int64_t max_scratch);
+void gram(const GramPlan &p, int64_t nrows, int m, int n, const double *X, int64_t ldx,
+          const double *Y, int64_t ldy, double *G, int ldg, bool lower_only, double *partial,
           hipStream_t s);
 // Y (nrows x n) = X (nrows x m) C (m x n) + Y diag(beta), with C given
 // transposed and k-major: Ct[k * ldct + j] = C[k][j] (device).
@@ -62,11 +75,15 @@ void gram(int64_t nrows, int m, int n, const double *X, int64_t ldx, const doubl
 void lincomb(int64_t nrows, int m, int n, const double *X, int64_t ldx, const double *Ct,
              int ldct, int beta_mode, double beta_scalar, const double *beta, double *Y,
              int64_t ldy, hipStream_t s);
-// The matrix-core implementations behind gram() and lincomb().
-int64_t gram_mfma_scratch(int64_t nrows, int m, int n, bool lower_only);
+// The matrix-core implementations behind gram() and lincomb().  The Gram
+// takes a row-chunking candidate (gram_mfma_choice, or one bounded by the
+// scratch a caller has) and its partial-sum size.
+constexpr int kGramMfmaCands = 10;
+int gram_mfma_choice(int64_t nrows, int m, int n, bool lower_only, bool *tune);
+int64_t gram_mfma_scratch(int64_t nrows, int m, int n, int choice);
 void gram_mfma(int64_t nrows, int m, int n, const double *X, int64_t ldx, const double *Y,
-               int64_t ldy, double *G, int ldg, bool lower_only, double *partial,
-               hipStream_t s);
+               int64_t ldy, double *G, int ldg, bool lower_only, int choice, bool tune,
+               double *partial, hipStream_t s);
 void lincomb_mfma(int64_t nrows, int m, int n, const double *X, int64_t ldx, const double *Ct,
                   int ldct, int beta_mode, double beta_scalar, const double *beta, double *Y,
                   int64_t ldy, hipStream_t s);

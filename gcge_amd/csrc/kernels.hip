@@ -706,19 +706,56 @@ void col_dots(int64_t nrows, int ncols, const double *X, int64_t ldx, const doub
 // the streaming kernel, with X and Y swapped when X is the thin side.
 static bool gram_is_thin(int m, int n) { return m <= 8 || n <= 8; }
 
-int64_t gram_scratch(int64_t nrows, int m, int n, bool lower_only)
+static int64_t thin_scratch(int64_t nrows, int m, int n)
 {
-    if (!gram_is_thin(m, n)) return gram_mfma_scratch(nrows, m, n, lower_only);
     ThinPlan p = (n <= 8) ? thin_plan(nrows, m, n) : thin_plan(nrows, n, m);
     return (int64_t)p.nchunk * p.mtiles * p.mt * p.nb;
 }
 
-void gram(int64_t nrows, int m, int n, const double *X, int64_t ldx, const double *Y,
-          int64_t ldy, double *G, int ldg, bool lower_only, double *partial, hipStream_t s)
+GramPlan gram_plan(int64_t nrows, int m, int n, bool lower_only)
+{
+    GramPlan p;
+    if (m <= 0 || n <= 0) return p;
+    if (gram_is_thin(m, n)) {
+        p.thin = true;
+        p.scratch = thin_scratch(nrows, m, n);
+        return p;
+    }
+    p.choice = gram_mfma_choice(nrows, m, n, lower_only, &p.tune);
+    p.scratch = gram_mfma_scratch(nrows, m, n, p.choice);
+    return p;
+}
+
+GramPlan gram_plan_bounded(int64_t nrows, int m, int n, bool lower_only, int64_t max_scratch)
+{
+    GramPlan p = gram_plan(nrows, m, n, lower_only);
+    p.tune = false;
+    if (p.scratch <= max_scratch || p.thin) {
+        if (p.scratch > max_scratch) p.scratch = -1;
+        return p;
+    }
+    // the candidate with the fewest chunks that fits
+    int best = -1;
+    int64_t best_s = 0;
+    for (int i = 0; i < kGramMfmaCands; ++i) {
+        const int64_t e = gram_mfma_scratch(nrows, m, n, i);
+        if (e <= max_scratch && (best < 0 || e < best_s)) {
+            best = i;
+            best_s = e;
+        }
+    }
+    p.choice = best;
+    p.scratch = best < 0 ? -1 : best_s;
+    return p;
+}
+
+void gram(const GramPlan &p, int64_t nrows, int m, int n, const double *X, int64_t ldx,
+          const double *Y, int64_t ldy, double *G, int ldg, bool lower_only, double *partial,
+          hipStream_t s)
 {
     if (m <= 0 || n <= 0) return;
-    if (!gram_is_thin(m, n)) {
-        gram_mfma(nrows, m, n, X, ldx, Y, ldy, G, ldg, lower_only, partial, s);
+    if (!p.thin) {
+        gram_mfma(nrows, m, n, X, ldx, Y, ldy, G, ldg, lower_only, p.choice, p.tune, partial, s);
         return;
     }
     // lower_only needs no special case here: the full block is computed

@@ -213,7 +213,7 @@ GramPlanM plan_with(int64_t nrows, int m, int n, int64_t min_rows, int64_t max_w
 // Tuner candidates (min_rows, max_wg); the default is index kGramDefault.
 constexpr int64_t kGramMinRows[5] = {64, 128, 256, 512, 1024};
 constexpr int64_t kGramMaxWg[2] = {512, 1024};
-constexpr int kGramCands = 10;
+constexpr int kGramCands = kGramMfmaCands;
 constexpr int kGramDefault = 3 * 2 + 0;  // 512 rows, 512 workgroups
 
 GramPlanM plan_cand(int64_t nrows, int m, int n, int i)
@@ -227,12 +227,6 @@ TuneKey gram_key(int64_t nrows, int m, int n, bool lower_only)
     return TuneKey{kTuneGram, ilog2_floor(nrows), tiles < 64 ? tiles : 64, lower_only ? 1 : 0};
 }
 
-GramPlanM plan(int64_t nrows, int m, int n, bool lower_only)
-{
-    int ch = tune_lookup(gram_key(nrows, m, n, lower_only));
-    if (ch < 0 || ch >= kGramCands) ch = kGramDefault;
-    return plan_cand(nrows, m, n, ch);
-}
 
 // ---------------------------------------------------------------------
 // LinearComb: Y (nrows x n) = X (nrows x m) C (m x n) + Y diag(beta),
@@ -356,9 +350,16 @@ __global__ __launch_bounds__(kWG) void k_lincomb_mfma(
 
 }  // namespace
 
-int64_t gram_mfma_scratch(int64_t nrows, int m, int n, bool lower_only)
+int gram_mfma_choice(int64_t nrows, int m, int n, bool lower_only, bool *tune)
 {
-    GramPlanM p = plan(nrows, m, n, lower_only);
+    const int ch = tune_lookup(gram_key(nrows, m, n, lower_only));
+    if (tune) *tune = ch < 0 && tune_enabled();
+    return (ch < 0 || ch >= kGramCands) ? kGramDefault : ch;
+}
+
+int64_t gram_mfma_scratch(int64_t nrows, int m, int n, int choice)
+{
+    const GramPlanM p = plan_cand(nrows, m, n, choice);
     return (int64_t)p.nchunk * p.tm * p.tn * kGT * kGT;
 }
 
@@ -379,15 +380,14 @@ void launch_gram(const GramPlanM &p, int64_t nrows, int m, int n, const double *
 }  // namespace
 
 void gram_mfma(int64_t nrows, int m, int n, const double *X, int64_t ldx, const double *Y,
-               int64_t ldy, double *G, int ldg, bool lower_only, double *partial,
-               hipStream_t s)
+               int64_t ldy, double *G, int ldg, bool lower_only, int choice, bool tune,
+               double *partial, hipStream_t s)
 {
     if (m <= 0 || n <= 0) return;
-    const TuneKey key = gram_key(nrows, m, n, lower_only);
-    const int ch = tune_lookup(key);
-    if (ch < 0 && tune_enabled()) {
+    if (tune) {
         // time every chunking on a private partial buffer large enough for
         // all of them (the caller sized its buffer for the default plan)
+        const TuneKey key = gram_key(nrows, m, n, lower_only);
         int64_t need = 0;
         for (int i = 0; i < kGramCands; ++i) {
             const GramPlanM q = plan_cand(nrows, m, n, i);
@@ -406,7 +406,7 @@ void gram_mfma(int64_t nrows, int m, int n, const double *X, int64_t ldx, const 
         HIPCHECK(hipFree(buf));
         return;
     }
-    launch_gram(plan(nrows, m, n, lower_only), nrows, m, n, X, ldx, Y, ldy, G, ldg, lower_only,
+    launch_gram(plan_cand(nrows, m, n, choice), nrows, m, n, X, ldx, Y, ldy, G, ldg, lower_only,
                 partial, s);
 }
 

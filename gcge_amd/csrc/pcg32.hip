@@ -690,285 +690,6 @@ __global__ __launch_bounds__(kWG) void k_update_p_rr(
     }
 }
 
-// ---------------------------------------------------------------------
-// Persistent BlockPCG for small systems: one cooperative launch runs every
-// inner iteration.  At a few thousand to tens of thousands of rows an
-// iteration moves a few MB, far less than the fixed cost of the five
-// dependent launches of the loop above, so the whole loop runs inside one
-// kernel.  Workgroup g owns a contiguous range of slices (rows); per
-// iteration
-//   p = r + beta p on its rows | barrier | w = (A + sigma B) p on its rows,
-//   partial p.w | barrier | alpha from all partials, d += alpha p,
-//   r -= alpha w, partial r.r | barrier | rho, stop flags
-// Every workgroup sums the per-workgroup partials in the same order, so all
-// hold the same alpha, rho and stop flags and leave the loop together.
-// The barrier waits are bounded: a wait that gives up sets the status flag,
-// every workgroup then leaves at its next barrier, and the host solves the
-// system again with the launch-per-step loop.
-// ---------------------------------------------------------------------
-constexpr int kPersistMaxG = 64;          // workgroups
-constexpr int kPersistTiles = 4;          // 8-column tiles (32 columns)
-constexpr int64_t kPersistMaxRows = 65536;
-constexpr unsigned kSpinLimit = 1u << 19;  // polls of a barrier (about 0.5 s)
-
-struct PersistArgs {
-    int64_t n, nslices;
-    const int64_t *slice_ptr;
-    const int *colA;
-    const float *valA;
-    const int *colB;
-    const float *valB;
-    float sigma;
-    int nc, ntile, max_iter;
-    float rate, tol;
-    float *P, *Wv, *D, *R;
-    const float *rho0, *init_res;
-    const int *active0;
-    float *part;        // [2][gridDim.x][ntile * 8]: p.w, then r.r
-    unsigned *bar;      // arrivals, generation (zero at launch)
-    int *status;        // gave up (0/1), iterations run
-};
-
-// Grid-wide barrier over gridDim.x co-resident workgroups.  Returns false
-// once any workgroup has given up waiting.
-__device__ bool grid_sync(unsigned *bar, int *status, unsigned &gen, int *s_ok)
-{
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        bool ok = true;
-        const unsigned my = gen;
-        __threadfence();  // this workgroup's stores before its arrival
-        const unsigned prev =
-            __hip_atomic_fetch_add(&bar[0], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (prev == gridDim.x - 1) {
-            __hip_atomic_store(&bar[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            __threadfence();
-            __hip_atomic_store(&bar[1], my + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        } else {
-            unsigned spins = 0;
-            while (__hip_atomic_load(&bar[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == my) {
-                if (++spins == kSpinLimit) {
-                    ok = false;
-                    break;
-                }
-                __builtin_amdgcn_s_sleep(2);
-            }
-        }
-        __threadfence();  // the others' stores before this workgroup reads
-        if (__hip_atomic_load(&status[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0)
-            ok = false;
-        if (!ok) __hip_atomic_store(&status[0], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        gen = my + 1;
-        *s_ok = ok ? 1 : 0;
-    }
-    __syncthreads();
-    return *s_ok != 0;
-}
-
-template <int W, bool TWO>
-__global__ __launch_bounds__(kWG) void k_pcg_persistent(PersistArgs a)
-{
-    constexpr int MC = kPersistTiles * kC;
-    __shared__ float s_rho[MC], s_rho_old[MC], s_alpha[MC], s_init[MC];
-    __shared__ int s_act[MC], s_tile_on[kPersistTiles], s_cont, s_ok;
-    __shared__ float red[(kWG / 64 + 1) * kC];
-    const unsigned G = gridDim.x;
-    const int64_t g = blockIdx.x;
-    const int ncp = a.ntile * kC;
-    const int tid = threadIdx.x;
-    for (int c = tid; c < ncp; c += kWG) {
-        s_rho[c] = a.rho0[c];
-        s_rho_old[c] = 0.f;
-        s_init[c] = a.init_res[c];
-        s_act[c] = a.active0[c];
-    }
-    __syncthreads();
-    if (tid == 0) {
-        int any = 0;
-        for (int t = 0; t < a.ntile; ++t) {
-            int on = 0;
-            for (int q = 0; q < kC; ++q) on |= s_act[t * kC + q];
-            s_tile_on[t] = on;
-            any |= on;
-        }
-        s_cont = any;
-    }
-    __syncthreads();
-    const int64_t spg = (a.nslices + G - 1) / G;
-    const int64_t s_beg = g * spg;
-    const int64_t s_end = s_beg + spg < a.nslices ? s_beg + spg : a.nslices;
-    const int64_t r_beg = s_beg * 64;
-    const int64_t r_end = s_end * 64 < a.n ? s_end * 64 : a.n;
-    const int wave = tid >> 6, lane = tid & 63;
-    const uint32_t tbytes = (uint32_t)(a.n * kC * (int64_t)sizeof(float));
-    unsigned gen = 0;
-    int it = 0;
-    while (it < a.max_iter && s_cont) {
-        // p = r + beta p
-        for (int t = 0; t < a.ntile; ++t) {
-            if (!s_tile_on[t]) continue;  // uniform
-            float beta[kC];
-#pragma unroll
-            for (int q = 0; q < kC; ++q) {
-                const int c = t * kC + q;
-                beta[q] = (it == 0 || s_rho_old[c] == 0.f) ? 0.f : s_rho[c] / s_rho_old[c];
-            }
-            const int64_t o = (int64_t)t * a.n * kC;
-            for (int64_t i = r_beg + tid; i < r_end; i += kWG) {
-                const f8 r = load8(a.R + o + i * kC);
-                f8 p;
-                if (it == 0) {
-                    p = r;
-                } else {
-                    p = load8(a.P + o + i * kC);
-#pragma unroll
-                    for (int q = 0; q < kC; ++q) p.v[q] = r.v[q] + beta[q] * p.v[q];
-                }
-                store8(a.P + o + i * kC, p);
-            }
-        }
-        if (!grid_sync(a.bar, a.status, gen, &s_ok)) break;
-        // w = (A + sigma B) p on this workgroup's slices, partial p.w
-        float dot[kPersistTiles][kC];
-#pragma unroll
-        for (int t = 0; t < kPersistTiles; ++t)
-#pragma unroll
-            for (int q = 0; q < kC; ++q) dot[t][q] = 0.f;
-        for (int64_t sl = s_beg + wave; sl < s_end; sl += kWG / 64) {  // wave-uniform
-            const int64_t base = a.slice_ptr[sl];
-            const int w = __builtin_amdgcn_readfirstlane((int)((a.slice_ptr[sl + 1] - base) >> 6));
-            constexpr int WB = TWO ? W : 1;
-            uint32_t oa[W], ob[WB];
-            float va[W], vb[WB];
-#pragma unroll
-            for (int j = 0; j < W; ++j) {
-                if (j < w) {  // wave-uniform; padding covers every lane
-                    const int64_t e = base + (int64_t)j * 64 + lane;
-                    oa[j] = (uint32_t)a.colA[e] * (uint32_t)(kC * sizeof(float));
-                    if constexpr (TWO) {
-                        va[j] = a.valA[e];
-                        vb[j] = a.sigma * a.valB[e];
-                        ob[j] = (uint32_t)a.colB[e] * (uint32_t)(kC * sizeof(float));
-                    } else {
-                        va[j] = a.valB ? a.valA[e] + a.sigma * a.valB[e] : a.valA[e];
-                    }
-                }
-            }
-            const int64_t row = sl * 64 + lane;
-            const bool live = row < a.n;
-#pragma unroll
-            for (int t = 0; t < kPersistTiles; ++t) {
-                if (t >= a.ntile || !s_tile_on[t]) continue;  // uniform
-                const float *pt = a.P + (int64_t)t * a.n * kC;
-                const auto pr_rsrc = buffer_rsrc(pt, tbytes);
-                f8 acc;
-#pragma unroll
-                for (int q = 0; q < kC; ++q) acc.v[q] = 0.f;
-                for_width<W>(w, [&](auto kw) {
-                    constexpr int K = decltype(kw)::value;
-#pragma unroll
-                    for (int j = 0; j < K; ++j) {
-                        const f8 pa = buffer_load8(pr_rsrc, oa[j]);
-#pragma unroll
-                        for (int q = 0; q < kC; ++q) acc.v[q] += va[j] * pa.v[q];
-                        if constexpr (TWO) {
-                            const f8 pb = buffer_load8(pr_rsrc, ob[j]);
-#pragma unroll
-                            for (int q = 0; q < kC; ++q) acc.v[q] += vb[j] * pb.v[q];
-                        }
-                    }
-                });
-                if (live) {
-                    store8(a.Wv + ((int64_t)t * a.n + row) * kC, acc);
-                    const f8 pr = load8(pt + row * kC);
-#pragma unroll
-                    for (int q = 0; q < kC; ++q) dot[t][q] += acc.v[q] * pr.v[q];
-                }
-            }
-        }
-        float *part_pw = a.part + (int64_t)g * ncp;
-#pragma unroll
-        for (int t = 0; t < kPersistTiles; ++t) {
-            if (t >= a.ntile) break;  // uniform
-            block_sum8(dot[t], red);
-            if (tid < kC) part_pw[t * kC + tid] = red[kWG / 64 * kC + tid];
-            __syncthreads();
-        }
-        if (!grid_sync(a.bar, a.status, gen, &s_ok)) break;
-        // alpha = rho / p.w, then d += alpha p, r -= alpha w, partial r.r
-        for (int c = tid; c < ncp; c += kWG) {
-            double sum = 0.0;
-            for (unsigned q = 0; q < G; ++q) sum += a.part[(int64_t)q * ncp + c];
-            s_alpha[c] = s_act[c] ? (float)((double)s_rho[c] / sum) : 0.f;
-        }
-        __syncthreads();
-        float rr[kPersistTiles][kC];
-#pragma unroll
-        for (int t = 0; t < kPersistTiles; ++t)
-#pragma unroll
-            for (int q = 0; q < kC; ++q) rr[t][q] = 0.f;
-#pragma unroll
-        for (int t = 0; t < kPersistTiles; ++t) {
-            if (t >= a.ntile || !s_tile_on[t]) continue;  // uniform
-            float al[kC];
-#pragma unroll
-            for (int q = 0; q < kC; ++q) al[q] = s_alpha[t * kC + q];
-            const int64_t o = (int64_t)t * a.n * kC;
-            for (int64_t i = r_beg + tid; i < r_end; i += kWG) {
-                const f8 p = load8(a.P + o + i * kC), w = load8(a.Wv + o + i * kC);
-                f8 d = load8(a.D + o + i * kC), r = load8(a.R + o + i * kC);
-#pragma unroll
-                for (int q = 0; q < kC; ++q) {
-                    d.v[q] += al[q] * p.v[q];
-                    r.v[q] -= al[q] * w.v[q];
-                    rr[t][q] += r.v[q] * r.v[q];
-                }
-                store8(a.D + o + i * kC, d);
-                store8(a.R + o + i * kC, r);
-            }
-        }
-        float *part_rr = a.part + ((int64_t)G + g) * ncp;
-#pragma unroll
-        for (int t = 0; t < kPersistTiles; ++t) {
-            if (t >= a.ntile) break;  // uniform
-            block_sum8(rr[t], red);
-            if (tid < kC) part_rr[t * kC + tid] = red[kWG / 64 * kC + tid];
-            __syncthreads();
-        }
-        if (!grid_sync(a.bar, a.status, gen, &s_ok)) break;
-        // rho and stop flags, the same in every workgroup
-        for (int c = tid; c < ncp; c += kWG) {
-            if (s_act[c]) {
-                double sum = 0.0;
-                for (unsigned q = 0; q < G; ++q) sum += a.part[((int64_t)G + q) * ncp + c];
-                const float res = (float)std::sqrt(sum);
-                s_rho_old[c] = s_rho[c];
-                s_rho[c] = (float)sum;
-                s_act[c] = (res > a.rate * s_init[c] && res > a.tol) ? 1 : 0;
-            }
-        }
-        __syncthreads();
-        if (tid == 0) {
-            int any = 0;
-            for (int t = 0; t < a.ntile; ++t) {
-                int on = 0;
-                for (int q = 0; q < kC; ++q) on |= s_act[t * kC + q];
-                s_tile_on[t] = on;
-                any |= on;
-            }
-            s_cont = any;
-        }
-        __syncthreads();
-        ++it;
-    }
-    if (g == 0 && tid == 0) a.status[1] = it;
-}
-
-__global__ void k_zero_ints(int *p, int count)
-{
-    if ((int)threadIdx.x < count) p[threadIdx.x] = 0;
-}
-
 struct PCG32 {
     int max_iter = 30;
     double rate = 1e-2, tol = 1e-14;
@@ -988,12 +709,7 @@ struct PCG32 {
     int *tiles_b = nullptr, *ntact_b = nullptr, *active_b = nullptr;
     int *h_ntact = nullptr;
     int64_t iters = 0;
-    // persistent loop: partial sums, barrier words and status
-    bool use_persistent = false;
     bool use_fused = true;  // three launches per iteration up to kFuseMaxRows rows
-    float *ppart = nullptr;
-    int *pctl = nullptr, *h_pstat = nullptr;
-    int64_t persistent_calls = 0, persistent_fallbacks = 0;
     // kCheck inner iterations captured as one graph, for one key
     bool use_graphs = true;
     hipGraphExec_t gexec = nullptr;
@@ -1050,11 +766,6 @@ struct PCG32 {
         HIPCHECK(hipMalloc(reinterpret_cast<void **>(&ntact_b), sizeof(int)));
         HIPCHECK(hipMalloc(reinterpret_cast<void **>(&partial_rr),
                            sizeof(float) * (size_t)ncap * 1024));
-        HIPCHECK(hipMalloc(reinterpret_cast<void **>(&ppart),
-                           sizeof(float) * 2 * kPersistMaxG * kPersistTiles * kC));
-        HIPCHECK(hipMalloc(reinterpret_cast<void **>(&pctl), sizeof(int) * 4));
-        HIPCHECK(hipHostMalloc(reinterpret_cast<void **>(&h_pstat), sizeof(int) * 2,
-                               hipHostMallocDefault));
     }
     int64_t nparts_max() const { return (n + kWG - 1) / kWG; }
     void release()
@@ -1075,9 +786,6 @@ struct PCG32 {
         HIPCHECK(hipFree(tiles_b));
         HIPCHECK(hipFree(ntact_b));
         HIPCHECK(hipFree(partial_rr));
-        HIPCHECK(hipFree(ppart));
-        HIPCHECK(hipFree(pctl));
-        HIPCHECK(hipHostFree(h_pstat));
         x = nullptr;
     }
 };
@@ -1089,16 +797,7 @@ unsigned blocks_for(int64_t n, int cap)
     return (unsigned)(b < 1 ? 1 : b);
 }
 
-void pcg32_run(void *matA, void **mv_b, void **mv_x, int *start, int *end, OPS *ops,
-               bool allow_persistent);
-
 void pcg32_solve(void *matA, void **mv_b, void **mv_x, int *start, int *end, OPS *ops)
-{
-    pcg32_run(matA, mv_b, mv_x, start, end, ops, true);
-}
-
-void pcg32_run(void *matA, void **mv_b, void **mv_x, int *start, int *end, OPS *ops,
-               bool allow_persistent)
 {
     PCG32 &S = *static_cast<PCG32 *>(ops->multi_linear_solver_workspace);
     Mat *A = mat_of(matA, "BlockPCG32 A");
@@ -1188,69 +887,6 @@ void pcg32_run(void *matA, void **mv_b, void **mv_x, int *start, int *end, OPS *
     // writes p; d += alpha p, r -= alpha w reads p, w, d, r and writes d, r.
     const double tile_vec = 4.0 * kC * (double)n;
     const float *vB = sigma != 0.f ? Bs->val32 : nullptr;
-    // small systems, when asked for: the whole loop in one cooperative
-    // launch.  Only while this thread's context is the process's only one
-    // (held until the launch has finished): a run of the concurrent solve
-    // service with this path on ended in a host segmentation fault whose
-    // cause was not found, and the path is slower than the loop below on
-    // MI355X (its grid barriers write back and invalidate L2).
-    std::unique_lock<std::mutex> solo;
-    if (allow_persistent && S.use_persistent && !c.prof.on && ntact > 0 &&
-        n <= kPersistMaxRows && ntile <= kPersistTiles && A->sell.max_width <= 8)
-        solo = sole_context_lock();
-    if (solo.owns_lock()) {
-        PersistArgs pa;
-        pa.n = n;
-        pa.nslices = A->sell.nslices;
-        pa.slice_ptr = A->sell.slice_ptr;
-        pa.colA = A->sell.col;
-        pa.valA = A->sell.val32;
-        pa.colB = Bs->col;
-        pa.valB = vB;
-        pa.sigma = sigma;
-        pa.nc = nc;
-        pa.ntile = ntile;
-        pa.max_iter = S.max_iter;
-        pa.rate = (float)S.rate;
-        pa.tol = (float)S.tol;
-        pa.P = S.p;
-        pa.Wv = S.w;
-        pa.D = S.x;
-        pa.R = S.r;
-        pa.rho0 = S.rho;
-        pa.init_res = S.init_res;
-        pa.active0 = S.active;
-        pa.part = S.ppart;
-        pa.bar = reinterpret_cast<unsigned *>(S.pctl);
-        pa.status = S.pctl + 2;
-        k_zero_ints<<<1, 64, 0, st>>>(S.pctl, 4);
-        const int64_t ng = (pa.nslices + 3) / 4;  // at least one slice per wavefront
-        const unsigned G = (unsigned)(ng < kPersistMaxG ? (ng < 1 ? 1 : ng) : kPersistMaxG);
-        void *args[] = {&pa};
-        const void *fn = two ? reinterpret_cast<const void *>(&k_pcg_persistent<8, true>)
-                             : reinterpret_cast<const void *>(&k_pcg_persistent<8, false>);
-        const hipError_t e = hipLaunchCooperativeKernel(fn, dim3(G), dim3(kWG), args, 0, st);
-        if (e == hipSuccess) {
-            HIPCHECK(hipMemcpyAsync(S.h_pstat, S.pctl + 2, sizeof(int) * 2,
-                                    hipMemcpyDeviceToHost, st));
-            c.sync();
-            if (S.h_pstat[0] == 0) {
-                S.persistent_calls += 1;
-                S.iters += S.h_pstat[1];
-                k_add_up<<<dim3(nblk_row, ntile), kWG, 0, st>>>(
-                    n, nc, S.x, Xv->d + (int64_t)start[1] * Xv->ldd, Xv->ldd);
-                return;
-            }
-        } else {
-            (void)hipGetLastError();  // launch refused: clear it and use the loop below
-        }
-        // a barrier wait gave up, or the launch was refused: solve again
-        // from the start with one launch per step
-        S.persistent_fallbacks += 1;
-        solo.unlock();
-        pcg32_run(matA, mv_b, mv_x, start, end, ops, false);
-        return;
-    }
     auto spmm_pw = [&](bool x, const int *tl, const int *nt_p) {
 #define SPMM_PW(W, TWO, XC)                                                                   \
     k_spmm_pw<W, TWO, XC><<<nblk_row, kWG, 0, st>>>(n, A->sell.slice_ptr, A->sell.col,        \
@@ -1420,6 +1056,14 @@ void pcg32_run(void *matA, void **mv_b, void **mv_x, int *start, int *end, OPS *
 }
 
 }  // namespace
+
+int64_t pcg32_iterations(OPS *ops)
+{
+    if (ops->MultiLinearSolver != pcg32_solve || ops->multi_linear_solver_workspace == nullptr)
+        return 0;
+    return static_cast<const PCG32 *>(ops->multi_linear_solver_workspace)->iters;
+}
+
 }  // namespace gcge
 
 extern "C" void HIPGCG_MultiLinearSolverFree_BlockPCG32(OPS *ops)
@@ -1434,14 +1078,6 @@ extern "C" void HIPGCG_MultiLinearSolverFree_BlockPCG32(OPS *ops)
     ops->MultiLinearSolver = nullptr;
 }
 
-extern "C" void HIPGCG_BlockPCG32_SetPersistent(OPS *ops, int on)
-{
-    using namespace gcge;
-    if (ops->MultiLinearSolver != pcg32_solve || ops->multi_linear_solver_workspace == nullptr)
-        fail("HIPGCG_BlockPCG32_SetPersistent: no device BlockPCG installed", __FILE__, __LINE__);
-    static_cast<PCG32 *>(ops->multi_linear_solver_workspace)->use_persistent = on != 0;
-}
-
 extern "C" void HIPGCG_BlockPCG32_SetFused(OPS *ops, int on)
 {
     using namespace gcge;
@@ -1450,16 +1086,6 @@ extern "C" void HIPGCG_BlockPCG32_SetFused(OPS *ops, int on)
     PCG32 *S = static_cast<PCG32 *>(ops->multi_linear_solver_workspace);
     S->use_fused = on != 0;
     S->drop_graph();
-}
-
-extern "C" void HIPGCG_BlockPCG32_Counts(OPS *ops, int64_t *persistent, int64_t *fallbacks)
-{
-    using namespace gcge;
-    if (ops->MultiLinearSolver != pcg32_solve || ops->multi_linear_solver_workspace == nullptr)
-        fail("HIPGCG_BlockPCG32_Counts: no device BlockPCG installed", __FILE__, __LINE__);
-    const PCG32 *S = static_cast<const PCG32 *>(ops->multi_linear_solver_workspace);
-    if (persistent) *persistent = S->persistent_calls;
-    if (fallbacks) *fallbacks = S->persistent_fallbacks;
 }
 
 extern "C" void HIPGCG_BlockPCG32_SetGraphs(OPS *ops, int on)

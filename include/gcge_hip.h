@@ -232,7 +232,9 @@ void HIPGCG_Destroy(void ***mv_ws, double *dbl_ws, int *int_ws, int nevMax,
  * device BlockPCG workspace when one is installed. */
 void HIPGCG_Free(OPS *ops);
 /* Results of the last solve: stats[0] numIter, stats[1] nevConv,
- * stats[2] total BlockPCG iterations, stats[3] linear solves;
+ * stats[2] total BlockPCG iterations (of the driver's FP64 BlockPCG and of
+ * a device FP32 BlockPCG in the MultiLinearSolver slot), stats[3] linear
+ * solves;
  * times[0..9] phase seconds (initX, checkconv, compP, compRR, rr_matW,
  * dsyevx, compRV, compW, compX, linsol; the names of
  * src/ops_eig_sol_gcg.c:27-41). */
@@ -288,21 +290,11 @@ void HIPGCG_MultiLinearSolverFree_BlockPCG32(OPS *ops);
  * captured HIP graph (default on; off while kernel timing is enabled).  No
  * counterpart in the reference; for callers that need plain stream launches. */
 void HIPGCG_BlockPCG32_SetGraphs(OPS *ops, int on);
-/* With on != 0, systems of at most 65,536 rows with rows of at most 8
- * entries and at most 32 columns run the whole BlockPCG loop in one
- * cooperative launch with grid-wide barriers (default off; never while
- * kernel timing is enabled or while the process has more than one library
- * context).  It is slower than the launch-per-step loop on MI355X.  A
- * barrier wait that gives up, or a refused launch, makes the call solve
- * again with one launch per step.  Counts: calls that ran in one launch,
- * calls that fell back.  No counterpart in the reference. */
-void HIPGCG_BlockPCG32_SetPersistent(OPS *ops, int on);
 /* Up to 65,536 rows an inner iteration is three launches, the two
  * single-workgroup reductions being done by every workgroup of the kernel
  * that follows each (default on; same sums, so the same bits as the
  * five-launch iteration).  No counterpart in the reference. */
 void HIPGCG_BlockPCG32_SetFused(OPS *ops, int on);
-void HIPGCG_BlockPCG32_Counts(OPS *ops, int64_t *persistent, int64_t *fallbacks);
 
 /* ------------------------------------------------------------------ */
 /* Solve service: concurrent requests on one GPU                       */

@@ -87,15 +87,6 @@ def inner_prod(nsd: str, X: np.ndarray, Y: np.ndarray) -> np.ndarray:
     return G
 
 
-def store_block(ip: np.ndarray, G: np.ndarray, nsd: str, ld_rows: int | None = None):
-    """Write G into the host array `ip` as the reference does with ldIP:
-    full block for 'N'/'S', the diagonal with stride ldIP for 'D'."""
-    if nsd == "D":
-        ip[: G.size] = G
-    else:
-        ip[: G.shape[0], : G.shape[1]] = G
-
-
 def axpby(alpha: float, X: np.ndarray | None, beta: float, Y: np.ndarray) -> None:
     """Y <- alpha X + beta Y in place; beta == 0 overwrites (the memset of
     app/app_lapack.c:350-351), X None scales only."""

@@ -8,6 +8,12 @@ for small problems, the dense LAPACK app (app/app_lapack.c).
 
 Run in the build container (the reference sources and MKL are needed to
 build oracle/_ref):  python tests/golden/make_golden.py
+
+With --cfg2 it instead writes tests/golden/ref_cfg2.json: one run of the
+OpenMP build (oracle/_ref/libgcge_ref_omp.so, OMP_NUM_THREADS threads,
+MKL one thread each) on the 128^3 7-point problem with nev 100, block 120,
+nevInit = nevMax = 220, autoshift on, srand(0) before the solve as
+gcge.c:508 does.  It takes about half an hour on 8 cores.
 """
 import json
 import os
@@ -54,7 +60,30 @@ def problem(name, n, dim, nev):
     return P.CCSProblem(name, A, B, nev, bs, ni, nm)
 
 
+def main_cfg2():
+    """The headline problem through the reference's CCS app (app/app_ccs.c),
+    its GCG driver (src/ops_eig_sol_gcg.c:1259-1562) and the plug-in's call
+    sequence (gcge.c:442-535, via oracle/ref_harness.c)."""
+    from ref_binding import DEFAULT_OMP
+    ref = RefLib(DEFAULT_OMP)
+    prob = P.config2()
+    r = ref.solve_ccs(prob, shift=1)
+    rec = {"name": prob.name, "n": 128, "dim": 3, "nev": prob.nev, "shift": 1, "app": "ccs",
+           "block_size": prob.block_size, "nev_init": prob.nev_init, "nev_max": prob.nev_max,
+           "tol": [prob.tol_abs, prob.tol_rel], "gap_min": prob.gap_min,
+           "num_iter": r["num_iter"], "nev_conv": r["nev_conv"], "seconds": r["seconds"],
+           "threads": os.environ.get("OMP_NUM_THREADS"),
+           "eval": [float(x) for x in r["eval"][:prob.nev_max]]}
+    print(prob.name, r["num_iter"], r["nev_conv"], r["seconds"], flush=True)
+    with open(os.path.join(HERE, "ref_cfg2.json"), "w") as f:
+        json.dump({"source": "reference GCGE compiled from its sources by oracle/Makefile "
+                             "(OpenMP variant), MKL 2021.4 BLAS/LAPACK, one process, no MPI",
+                   "run": rec}, f, indent=1)
+
+
 def main():
+    if "--cfg2" in sys.argv:
+        return main_cfg2()
     ref = RefLib()
     out = []
     for name, n, dim, nev, shift, apps in CASES:

@@ -21,8 +21,7 @@ def ia(*v):
     return (C.c_int * len(v))(*v)
 
 
-def run(table, lib, A, dA, dB, sigma, X0, Bm, max_iter, rate, ncols, b_off=3, x_off=5,
-        persistent=True):
+def run(table, lib, A, dA, dB, sigma, X0, Bm, max_iter, rate, ncols, b_off=3, x_off=5):
     n = A.shape[0]
     rng = np.random.default_rng(ncols)
     b = rng.standard_normal((n, ncols + b_off))
@@ -33,7 +32,6 @@ def run(table, lib, A, dA, dB, sigma, X0, Bm, max_iter, rate, ncols, b_off=3, x_
     table.upload(hx, x)
     lib.HIPGCG_MultiLinearSolverSetup_BlockPCG32(max_iter, rate, 0.0,
                                                  dB.handle if dB else None, sigma, table.ops)
-    lib.HIPGCG_BlockPCG32_SetPersistent(table.ops, 1 if persistent else 0)
     abi.slot(table.ops, "MultiLinearSolver")(dA.handle, hb, hx, ia(b_off, x_off),
                                              ia(b_off + ncols, x_off + ncols), table.ops)
     xs = table.download(hx, 0, x.shape[1], n)
@@ -54,12 +52,10 @@ def setup(hip):
     return HipTable(), hip, A, B, DeviceMatrix(A, fp32=True), DeviceMatrix(B, fp32=True)
 
 
-@pytest.mark.parametrize("persistent", [True, False])
 @pytest.mark.parametrize("ncols", [1, 5, 9, 33])
-def test_converges_to_fp32_accuracy(setup, ncols, persistent):
+def test_converges_to_fp32_accuracy(setup, ncols):
     table, lib, A, B, dA, dB = setup
-    rel, xx = run(table, lib, A, dA, None, 0.0, None, None, 2000, 1e-6, ncols,
-                  persistent=persistent)
+    rel, xx = run(table, lib, A, dA, None, 0.0, None, None, 2000, 1e-6, ncols)
     # FP32 recursion: the true residual trails the recursive one by about
     # cond(A) * eps32 (cond ~ 1e3 here)
     assert rel.max() <= 1e-4, rel.max()
@@ -71,32 +67,28 @@ def test_converges_to_fp32_accuracy(setup, ncols, persistent):
     assert np.max(np.abs(xx - xe)) <= 1e-3 * np.max(np.abs(xe))
 
 
-@pytest.mark.parametrize("persistent", [True, False])
-def test_rate_stop(setup, persistent):
+def test_rate_stop(setup):
     """With rate 1e-2 every column stops once its residual has dropped 100x
     (and not far beyond: it stops at the first iteration below)."""
     table, lib, A, B, dA, dB = setup
-    rel, _ = run(table, lib, A, dA, None, 0.0, None, None, 500, 1e-2, 12,
-                 persistent=persistent)
+    rel, _ = run(table, lib, A, dA, None, 0.0, None, None, 500, 1e-2, 12)
     assert rel.max() <= 1e-2
     assert rel.min() > 1e-5
 
 
-@pytest.mark.parametrize("persistent", [True, False])
-def test_max_iter_stop(setup, persistent):
+def test_max_iter_stop(setup):
     table, lib, A, B, dA, dB = setup
-    rel1, _ = run(table, lib, A, dA, None, 0.0, None, None, 1, 1e-12, 4, persistent=persistent)
-    rel5, _ = run(table, lib, A, dA, None, 0.0, None, None, 5, 1e-12, 4, persistent=persistent)
+    rel1, _ = run(table, lib, A, dA, None, 0.0, None, None, 1, 1e-12, 4)
+    rel5, _ = run(table, lib, A, dA, None, 0.0, None, None, 5, 1e-12, 4)
     assert np.all(rel1 < 1.0) and np.all(rel5 < rel1)
 
 
-@pytest.mark.parametrize("persistent", [True, False])
-def test_shifted_operator(setup, persistent):
+def test_shifted_operator(setup):
     """(A + sigma B) x = b with B's FP32 copy; nonzero initial guess."""
     table, lib, A, B, dA, dB = setup
     n = A.shape[0]
     X0 = np.random.default_rng(9).standard_normal((n, 11))
-    rel, _ = run(table, lib, A, dA, dB, 0.37, X0, B, 2000, 1e-6, 6, persistent=persistent)
+    rel, _ = run(table, lib, A, dA, dB, 0.37, X0, B, 2000, 1e-6, 6)
     # FP32 recursion: the true residual trails the recursive one by about
     # cond(A) * eps32 (cond ~ 1e3 here)
     assert rel.max() <= 1e-4, rel.max()
@@ -116,7 +108,6 @@ def test_graph_replays_match_eager(setup, rate):
     hx = table.create_multivec(nc, dA)
     table.upload(hb, b)
     lib.HIPGCG_MultiLinearSolverSetup_BlockPCG32(30, rate, 0.0, None, 0.0, table.ops)
-    lib.HIPGCG_BlockPCG32_SetPersistent(table.ops, 0)  # the launch-per-step loop
     outs = []
     for graphs in (0, 1, 1, 0, 1, 1):
         lib.HIPGCG_BlockPCG32_SetGraphs(table.ops, graphs)
@@ -150,7 +141,6 @@ def test_row_block_orders_give_identical_bits(setup, tmp_path, shifted):
     lib.HIPGCG_MultiLinearSolverSetup_BlockPCG32(400, 1e-2, 0.0,
                                                  dB.handle if shifted else None, sigma,
                                                  table.ops)
-    lib.HIPGCG_BlockPCG32_SetPersistent(table.ops, 0)  # the launch-per-step loop
     lg = int(np.floor(np.log2(n)))
     outs = []
     try:
@@ -187,7 +177,6 @@ def test_graph_follows_new_parameters(setup):
         for rate, graphs in ((0.0, 1), (1e-2, 1), (1e-2, 0)):
             lib.HIPGCG_MultiLinearSolverSetup_BlockPCG32(300 if rate else 40, rate, 0.0, None,
                                                          0.0, table.ops)
-            lib.HIPGCG_BlockPCG32_SetPersistent(table.ops, 0)
             lib.HIPGCG_BlockPCG32_SetGraphs(table.ops, graphs)
             table.upload(hx, np.zeros((n, nc)))
             abi.slot(table.ops, "MultiLinearSolver")(dA.handle, hb, hx, ia(0, 0), ia(nc, nc),
@@ -200,47 +189,6 @@ def test_graph_follows_new_parameters(setup):
     assert np.array_equal(outs[(1e-2, 1)], outs[(1e-2, 0)])
     r = np.linalg.norm(b - A @ outs[(1e-2, 1)], axis=0) / np.linalg.norm(b, axis=0)
     assert r.max() < 1.01e-2
-
-
-@pytest.mark.parametrize("shifted", [False, True])
-@pytest.mark.parametrize("n_side", [40, 129])
-def test_persistent_loop_matches_launch_loop(hip, shifted, n_side):
-    """The one-launch loop (grid barriers) and the launch-per-step loop solve
-    the same systems to the same accuracy with the same stopping rule; the
-    one-launch path ran and never fell back.  129^2 = 16,641 rows: several
-    slices per workgroup and a partial last slice."""
-    A, B = P.stencil_pair(n_side, 2)
-    table = HipTable()
-    dA, dB = DeviceMatrix(A, fp32=True), DeviceMatrix(B, fp32=True)
-    n, nc = A.shape[0], 20
-    b = np.random.default_rng(45).standard_normal((n, nc))
-    hb = table.create_multivec(nc, dA)
-    hx = table.create_multivec(nc, dA)
-    table.upload(hb, b)
-    sigma = 0.37 if shifted else 0.0
-    out = {}
-    for persistent in (1, 0):
-        lib = hip
-        lib.HIPGCG_MultiLinearSolverSetup_BlockPCG32(400, 1e-3, 0.0,
-                                                     dB.handle if shifted else None, sigma,
-                                                     table.ops)
-        lib.HIPGCG_BlockPCG32_SetPersistent(table.ops, persistent)
-        table.upload(hx, np.zeros((n, nc)))
-        abi.slot(table.ops, "MultiLinearSolver")(dA.handle, hb, hx, ia(0, 0), ia(nc, nc),
-                                                 table.ops)
-        out[persistent] = table.download(hx, 0, nc, n)
-    pc, fb = C.c_int64(), C.c_int64()
-    hip.HIPGCG_BlockPCG32_Counts(table.ops, C.byref(pc), C.byref(fb))
-    table.destroy_multivec(hb, nc)
-    table.destroy_multivec(hx, nc)
-    assert pc.value == 1 and fb.value == 0
-    op = A + sigma * B
-    for x in out.values():
-        r = np.linalg.norm(b - op @ x, axis=0) / np.linalg.norm(b, axis=0)
-        assert r.max() <= 1.01e-3, r.max()
-    # different partial-sum grouping: the iterates agree to FP32 accuracy
-    d = np.linalg.norm(out[1] - out[0]) / np.linalg.norm(out[0])
-    assert d <= 1e-2, d
 
 
 @pytest.mark.parametrize("graphs", [0, 1])
@@ -260,7 +208,6 @@ def test_three_launch_iteration_matches_five(setup, graphs, rate):
     try:
         for fused in (1, 0, 1):
             lib.HIPGCG_MultiLinearSolverSetup_BlockPCG32(81, rate, 0.0, None, 0.0, table.ops)
-            lib.HIPGCG_BlockPCG32_SetPersistent(table.ops, 0)
             lib.HIPGCG_BlockPCG32_SetGraphs(table.ops, graphs)
             lib.HIPGCG_BlockPCG32_SetFused(table.ops, fused)
             table.upload(hx, np.zeros((n, nc)))
@@ -283,7 +230,7 @@ def test_five_launch_iteration_above_threshold(hip):
     A, B = P.stencil_pair(264, 2)  # 69,696 rows
     table = HipTable()
     dA = DeviceMatrix(A, fp32=True)
-    rel, _ = run(table, hip, A, dA, None, 0.0, None, None, 400, 1e-2, 9, persistent=False)
+    rel, _ = run(table, hip, A, dA, None, 0.0, None, None, 400, 1e-2, 9)
     assert rel.max() <= 1.01e-2
 
 
@@ -306,7 +253,6 @@ def test_block_columns_equal_columns_solved_alone(hip, graphs):
     table.upload(hb, b)
     try:
         hip.HIPGCG_MultiLinearSolverSetup_BlockPCG32(400, 1e-2, 0.0, None, 0.0, table.ops)
-        hip.HIPGCG_BlockPCG32_SetPersistent(table.ops, 0)
         hip.HIPGCG_BlockPCG32_SetGraphs(table.ops, graphs)
         solve = abi.slot(table.ops, "MultiLinearSolver")
         table.upload(hx, np.zeros((n, nc)))

@@ -317,3 +317,36 @@ def test_analytic_spot_values():
         A, B = P.stencil_pair(n, dim)
         ex = O.dense_generalized_eigenvalues(A, B, 10)
         assert np.allclose(ex, P.analytic_eigenvalues(n, dim, 10), rtol=1e-12)
+
+
+def _sine_mode(n, i, j):
+    """The (i, j) sine mode of the n x n Dirichlet grid, point x + n y: an
+    eigenvector of the 5-point Laplacian (4 - mu) and of the mass matrix
+    ((8 + mu) / 12) alike, mu = 2 cos(i pi/(n+1)) + 2 cos(j pi/(n+1))."""
+    t = np.arange(1, n + 1) * np.pi / (n + 1)
+    v = np.outer(np.sin(j * t), np.sin(i * t)).ravel()  # row y, column x
+    mu = 2 * np.cos(i * np.pi / (n + 1)) + 2 * np.cos(j * np.pi / (n + 1))
+    return v, mu
+
+
+def test_relative_residuals_known_values():
+    """The residual checker against residuals fixed by the closed-form
+    eigenvectors: an exact eigenpair has residual at rounding level, and
+    x = v + eps u (u another mode) with the eigenvalue of v has
+    A x - lam B x = eps (lam_u - lam) b_u u, so its relative residual is
+    eps |lam_u - lam| b_u |u| / |lam| with b_u = (8 + mu_u) / 12.  A dropped
+    B, a wrong sign or a missing division by |lam| each fails."""
+    n = 24
+    A, B = P.stencil_pair(n, 2)
+    v, mv = _sine_mode(n, 1, 2)
+    u, mu_ = _sine_mode(n, 3, 1)
+    lam_v = (4 - mv) * 12 / (8 + mv)
+    lam_u = (4 - mu_) * 12 / (8 + mu_)
+    b_u = (8 + mu_) / 12
+    eps = 1e-3
+    X = np.stack([v, v + eps * u], axis=1)
+    lam = np.array([lam_v, lam_v])
+    rr = O.relative_residuals(A, B, lam, X)
+    assert rr[0] < 1e-13
+    expect = eps * abs(lam_u - lam_v) * b_u * np.linalg.norm(u) / lam_v
+    assert abs(rr[1] - expect) <= 1e-10 * expect, (rr[1], expect)
