@@ -156,6 +156,8 @@ EXPORTS = {
                            C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
     "HIP_MultiVecSetFromHost": (None, [C.c_void_p, C.c_int, C.c_int, c_dbl_p, C.c_int64]),
     "HIP_MultiVecGetToHost": (None, [C.c_void_p, C.c_int, C.c_int, c_dbl_p, C.c_int64]),
+    "HIP_MultiVecInnerProdNoAlloc": (C.c_int, [C.c_char, C.c_void_p, C.c_void_p, C.c_int, c_int_p,
+                                               c_int_p, c_dbl_p, C.c_int]),
     "HIP_MultiVecInfo": (None, [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64),
                                 C.POINTER(C.c_int64)]),
     "gcge_hip_set_random_mode": (None, [C.c_int, C.c_uint64]),

@@ -57,14 +57,15 @@ void col_dots(int64_t nrows, int ncols, const double *X, int64_t ldx, const doub
 // the two cannot make the launch need more scratch than was sized.
 struct GramPlan {
     bool thin = false;    // streaming kernel (a side of at most 8 columns)
-    int choice = -1;      // matrix-core kernel: row-chunking candidate
+    int choice = -1;      // matrix-core kernel: row-chunking candidate, or -1:
+    int64_t max_chunks = 0;  // at most this many row chunks (choice -1)
     bool tune = false;    // time every chunking first, on a private buffer
     int64_t scratch = 0;  // doubles of partial sums (0 or less: nothing fits)
 };
 GramPlan gram_plan(int64_t nrows, int m, int n, bool lower_only);
 // A plan whose partial sums fit in max_scratch doubles and that never
-// tunes (the table's choice when it fits, else the one with the fewest row
-// chunks that does); scratch <= 0 when no chunking fits.
+// tunes (the table's choice when it fits, else as many row chunks as fit);
+// scratch <= 0 when not even one chunk fits.
 GramPlan gram_plan_bounded(int64_t nrows, int m, int n, bool lower_only, int64_t max_scratch);
 void gram(const GramPlan &p, int64_t nrows, int m, int n, const double *X, int64_t ldx,
           const double *Y, int64_t ldy, double *G, int ldg, bool lower_only, double *partial,
@@ -76,14 +77,16 @@ void lincomb(int64_t nrows, int m, int n, const double *X, int64_t ldx, const do
              int ldct, int beta_mode, double beta_scalar, const double *beta, double *Y,
              int64_t ldy, hipStream_t s);
 // The matrix-core implementations behind gram() and lincomb().  The Gram
-// takes a row-chunking candidate (gram_mfma_choice, or one bounded by the
-// scratch a caller has) and its partial-sum size.
+// takes a row-chunking candidate (gram_mfma_choice), or choice -1 with a
+// bound on the number of row chunks, and its partial-sum size.
 constexpr int kGramMfmaCands = 10;
 int gram_mfma_choice(int64_t nrows, int m, int n, bool lower_only, bool *tune);
-int64_t gram_mfma_scratch(int64_t nrows, int m, int n, int choice);
+int64_t gram_mfma_scratch(int64_t nrows, int m, int n, int choice, int64_t max_chunks = 0);
+// doubles of partial sums per row chunk
+int64_t gram_mfma_chunk_doubles(int m, int n);
 void gram_mfma(int64_t nrows, int m, int n, const double *X, int64_t ldx, const double *Y,
-               int64_t ldy, double *G, int ldg, bool lower_only, int choice, bool tune,
-               double *partial, hipStream_t s);
+               int64_t ldy, double *G, int ldg, bool lower_only, int choice, int64_t max_chunks,
+               bool tune, double *partial, hipStream_t s);
 void lincomb_mfma(int64_t nrows, int m, int n, const double *X, int64_t ldx, const double *Ct,
                   int ldct, int beta_mode, double beta_scalar, const double *beta, double *Y,
                   int64_t ldy, hipStream_t s);

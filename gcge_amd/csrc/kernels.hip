@@ -734,18 +734,12 @@ GramPlan gram_plan_bounded(int64_t nrows, int m, int n, bool lower_only, int64_t
         if (p.scratch > max_scratch) p.scratch = -1;
         return p;
     }
-    // the candidate with the fewest chunks that fits
-    int best = -1;
-    int64_t best_s = 0;
-    for (int i = 0; i < kGramMfmaCands; ++i) {
-        const int64_t e = gram_mfma_scratch(nrows, m, n, i);
-        if (e <= max_scratch && (best < 0 || e < best_s)) {
-            best = i;
-            best_s = e;
-        }
-    }
-    p.choice = best;
-    p.scratch = best < 0 ? -1 : best_s;
+    // as many row chunks as fit
+    const int64_t per = gram_mfma_chunk_doubles(m, n);
+    const int64_t fit = max_scratch / per;
+    p.choice = -1;
+    p.max_chunks = fit;
+    p.scratch = fit < 1 ? -1 : gram_mfma_scratch(nrows, m, n, -1, fit);
     return p;
 }
 
@@ -755,7 +749,8 @@ void gram(const GramPlan &p, int64_t nrows, int m, int n, const double *X, int64
 {
     if (m <= 0 || n <= 0) return;
     if (!p.thin) {
-        gram_mfma(nrows, m, n, X, ldx, Y, ldy, G, ldg, lower_only, p.choice, p.tune, partial, s);
+        gram_mfma(nrows, m, n, X, ldx, Y, ldy, G, ldg, lower_only, p.choice, p.max_chunks, p.tune,
+                  partial, s);
         return;
     }
     // lower_only needs no special case here: the full block is computed

@@ -357,9 +357,22 @@ int gram_mfma_choice(int64_t nrows, int m, int n, bool lower_only, bool *tune)
     return (ch < 0 || ch >= kGramCands) ? kGramDefault : ch;
 }
 
-int64_t gram_mfma_scratch(int64_t nrows, int m, int n, int choice)
+// choice -1: at most max_chunks row chunks of at least kGramMinRows[0] rows
+static GramPlanM plan_of(int64_t nrows, int m, int n, int choice, int64_t max_chunks)
 {
-    const GramPlanM p = plan_cand(nrows, m, n, choice);
+    if (choice >= 0) return plan_cand(nrows, m, n, choice);
+    const int64_t tiles = (int64_t)((m + kGT - 1) / kGT) * ((n + kGT - 1) / kGT);
+    return plan_with(nrows, m, n, kGramMinRows[0], (max_chunks < 1 ? 1 : max_chunks) * tiles);
+}
+
+int64_t gram_mfma_chunk_doubles(int m, int n)
+{
+    return (int64_t)((m + kGT - 1) / kGT) * ((n + kGT - 1) / kGT) * kGT * kGT;
+}
+
+int64_t gram_mfma_scratch(int64_t nrows, int m, int n, int choice, int64_t max_chunks)
+{
+    const GramPlanM p = plan_of(nrows, m, n, choice, max_chunks);
     return (int64_t)p.nchunk * p.tm * p.tn * kGT * kGT;
 }
 
@@ -380,8 +393,8 @@ void launch_gram(const GramPlanM &p, int64_t nrows, int m, int n, const double *
 }  // namespace
 
 void gram_mfma(int64_t nrows, int m, int n, const double *X, int64_t ldx, const double *Y,
-               int64_t ldy, double *G, int ldg, bool lower_only, int choice, bool tune,
-               double *partial, hipStream_t s)
+               int64_t ldy, double *G, int ldg, bool lower_only, int choice, int64_t max_chunks,
+               bool tune, double *partial, hipStream_t s)
 {
     if (m <= 0 || n <= 0) return;
     if (tune) {
@@ -406,8 +419,8 @@ void gram_mfma(int64_t nrows, int m, int n, const double *X, int64_t ldx, const 
         HIPCHECK(hipFree(buf));
         return;
     }
-    launch_gram(plan_cand(nrows, m, n, choice), nrows, m, n, X, ldx, Y, ldy, G, ldg, lower_only,
-                partial, s);
+    launch_gram(plan_of(nrows, m, n, choice, max_chunks), nrows, m, n, X, ldx, Y, ldy, G, ldg,
+                lower_only, partial, s);
 }
 
 void lincomb_mfma(int64_t nrows, int m, int n, const double *X, int64_t ldx, const double *Ct,

@@ -127,6 +127,23 @@ void HIP_MultiVecGetToHost(void **mv, int start, int end, double *dst, int64_t l
 /* Rows, columns and leading dimension of a multivector handle. */
 void HIP_MultiVecInfo(void **mv, int64_t *nrows, int64_t *ncols, int64_t *ldd);
 
+/* Allocation-free inner products.  No counterpart in the reference: this
+ * is for callers that cannot allocate device memory (code that must not
+ * call hipMalloc, such as tight serving loops or harnesses that forbid
+ * it).  The same semantics as the table's MultiVecInnerProd slot
+ * (app/app_lapack.c:299-321: nsdIP 'N' full block, 'S' symmetric, 'D'
+ * diagonal; inner_prod a host array with leading dimension ldIP), with
+ * the partial sums and the device result kept in a fixed 32 MiB array
+ * that each GPU holds until the process ends, instead of buffers the
+ * library allocates.  Results are the slot's bit for bit whenever the
+ * slot's launch plan fits the array; otherwise the rows are grouped in as
+ * many chunks as fit (a different summation grouping).  Returns 0, or -1 when the block or
+ * its partial sums do not fit the array, or the arguments are invalid
+ * (gcge_hip_last_error says which).  Waits for its stream before it
+ * returns; calls that overlap in time on one GPU race on the array. */
+int HIP_MultiVecInnerProdNoAlloc(char nsdIP, void **x, void **y, int is_vec, int *start,
+                                 int *end, double *inner_prod, int ldIP);
+
 /* Random fill of MultiVecSetRandomValue:
  *   GCGE_HIP_RAND_LIBC   (default) host libc rand()/(RAND_MAX+1), column by
  *                        column, uploaded -- the reference's sequence
