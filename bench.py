@@ -197,8 +197,9 @@ def rocprof_summary(d: str):
 
 # kernel class of the library profiler -> device kernel name prefix
 # (one profiler launch of these classes is one dispatch of one of the kernels)
-KERNEL_OF_CLASS = {"lincomb": ["k_lincomb_mfma"], "spmm64": ["k_spmm_sell"],
-                   "spmm32": ["k_spmm_pw"], "pcgvec": ["k_update_p", "k_update_xr"]}
+KERNEL_OF_CLASS = {"lincomb": ["k_lincomb_mfma"], "spmm64": ["k_spmm_sell", "k_stencil7"],
+                   "spmm32": ["k_spmm_pw", "k_stencil_pw"],
+                   "pcgvec": ["k_update_p", "k_update_xr"]}
 
 
 def _counter_means(path: str, counters=("FETCH_SIZE", "WRITE_SIZE")):

@@ -152,6 +152,7 @@ EXPORTS = {
     "gcge_tune_entry": (C.c_int, [C.c_int, c_int_p, c_int_p, c_int_p, c_int_p, c_int_p,
                                   c_dbl_p, c_dbl_p]),
     "HIPCCS_Equal": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "HIPCCS_Stencil": (C.c_int, [C.c_void_p]),
     "HIPCCS_Info": (None, [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64),
                            C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
     "HIP_MultiVecSetFromHost": (None, [C.c_void_p, C.c_int, C.c_int, c_dbl_p, C.c_int64]),
@@ -203,6 +204,7 @@ EXPORTS = {
 }
 
 GCGE_HIP_MAT_FP32 = 1
+GCGE_HIP_MAT_GENERAL = 2
 GCGE_GRID_LAPLACE = 0
 GCGE_GRID_MASS = 1
 GCGE_GRID_DIFFUSION = 2

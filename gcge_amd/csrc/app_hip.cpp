@@ -9,6 +9,7 @@
 // src/ops_multi_vec.c:351-411 (MultiVecQtAP).
 #include <hip/hip_runtime.h>
 
+#include <cmath>
 #include <cstdarg>
 #include <cstdio>
 #include <cstdlib>
@@ -388,9 +389,11 @@ void h_MatDotMultiVec(void *mat, void **x, void **y, int *start, int *end, OPS *
     Mat *A = mat_of(mat, "MatDotMultiVec");
     if (A->n != X->nrows || A->n != Y->nrows)
         fail("MatDotMultiVec: matrix and vector sizes differ", __FILE__, __LINE__);
-    // algorithmic bytes: CCS arrays once, X read once, Y written once
-    const double bytes = (double)A->sell.nnz_true * 12.0 + 4.0 * (double)(A->n + 1) +
-                         2.0 * 8.0 * (double)nc * (double)A->n;
+    // algorithmic bytes: CCS arrays once (none for a matrix-free stencil
+    // operator), X read once, Y written once
+    const double mat_bytes = A->sell.stencil_n > 0 ? 0.0
+                             : (double)A->sell.nnz_true * 12.0 + 4.0 * (double)(A->n + 1);
+    const double bytes = mat_bytes + 2.0 * 8.0 * (double)nc * (double)A->n;
     hipEvent_t ev = c.prof.begin(PROF_SPMM64, bytes, 2.0 * (double)A->sell.nnz_true * nc,
                                  c.stream);
     spmm(A->sell, xs, X->ldd, ys, Y->ldd, nc, c.stream);
@@ -693,6 +696,53 @@ void OPS_HIP_Set(OPS *ops)
     ops->DenseMatOrth = lap->DenseMatOrth;
 }
 
+// The grid side n when the CCS matrix is a constant-coefficient 7-point
+// operator on an n^3 Dirichlet grid (every row: its in-grid neighbours
+// x +- 1, y +- n, z +- n^2 and itself in ascending order, one value on the
+// diagonal and one on every neighbour), with those values; 0 otherwise.
+static int detect_stencil7(const gcge_ccs_mat *m, double *d, double *o)
+{
+    const int64_t N = m->nrows;
+    int64_t n = (int64_t)std::llround(std::cbrt((double)N));
+    if (n < 4 || n > kStencilMaxN || n * n * n != N) return 0;
+    const int *jc = m->j_col, *ir = m->i_row;
+    const double *v = m->data;
+    double dv = 0.0, ov = 0.0;
+    bool have_o = false;
+    for (int64_t r = 0; r < N; ++r) {
+        const int64_t x = r % n, y = (r / n) % n, z = r / (n * n);
+        int64_t want[7];
+        int k = 0;
+        if (z > 0) want[k++] = r - n * n;
+        if (y > 0) want[k++] = r - n;
+        if (x > 0) want[k++] = r - 1;
+        want[k++] = r;
+        if (x < n - 1) want[k++] = r + 1;
+        if (y < n - 1) want[k++] = r + n;
+        if (z < n - 1) want[k++] = r + n * n;
+        if (jc[r + 1] - jc[r] != k) return 0;
+        for (int j = 0; j < k; ++j) {
+            const int64_t e = jc[r] + j;
+            if (ir[e] != want[j]) return 0;
+            const double val = v[e];
+            if (want[j] == r) {
+                if (r == 0) dv = val;
+                else if (val != dv || std::signbit(val) != std::signbit(dv)) return 0;
+            } else {
+                if (!have_o) {
+                    ov = val;
+                    have_o = true;
+                } else if (val != ov || std::signbit(val) != std::signbit(ov)) {
+                    return 0;
+                }
+            }
+        }
+    }
+    *d = dv;
+    *o = ov;
+    return (int)n;
+}
+
 void *HIPCCS_Create(const gcge_ccs_mat *host, int flags)
 {
     if (host == nullptr || host->nrows <= 0 || host->nrows != host->ncols ||
@@ -763,7 +813,8 @@ void *HIPCCS_Create(const gcge_ccs_mat *host, int flags)
         M->bytes += (int64_t)(sizeof(float) * v32.size());
     }
     detect_plane(S, c.stream);
-    (void)c;
+    if (!(flags & GCGE_HIP_MAT_GENERAL))
+        S.stencil_n = detect_stencil7(host, &S.stencil_d, &S.stencil_o);
     return M;
 }
 
@@ -780,6 +831,8 @@ void HIPCCS_Destroy(void **mat)
     delete M;
     *mat = nullptr;
 }
+
+int HIPCCS_Stencil(void *mat) { return mat_of(mat, "HIPCCS_Stencil")->sell.stencil_n; }
 
 void HIPCCS_Info(void *mat, int64_t *nrows, int64_t *nnz, int64_t *padded, int64_t *bytes)
 {

@@ -338,6 +338,11 @@ void *HIPCCS_CreateGrid(const gcge_grid_op *op, int flags)
     HIPCHECK(hipStreamSynchronize(st));  // the host coefficients and sp may go now
     if (tmp) HIPCHECK(hipFree(tmp));
     detect_plane(S, st);
+    if (form == kConst7 && !(flags & GCGE_HIP_MAT_GENERAL) && n >= 4 && n <= kStencilMaxN) {
+        S.stencil_n = (int)n;
+        S.stencil_d = a.dval;
+        S.stencil_o = a.oval;
+    }
     return M;
 }
 

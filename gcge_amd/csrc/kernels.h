@@ -22,6 +22,12 @@ struct SellMat {
     // in-plane reach: the typical largest |col - row| below half a plane
     // (rows; 0 when unknown)
     int64_t halo = 0;
+    // Constant-coefficient 7-point operator on an n^3 grid (diagonal
+    // stencil_d, every in-grid neighbour stencil_o, point x + n y + n^2 z):
+    // stencil_n = n when the matrix is one and products may run matrix-free
+    // (stencil.hip), 0 otherwise.
+    int stencil_n = 0;
+    double stencil_d = 0.0, stencil_o = 0.0;
     uint64_t structure_hash = 0;   // of the slice widths, to match A and B
     int64_t *slice_ptr = nullptr;  // nslices + 1
     int *col = nullptr;
@@ -32,6 +38,22 @@ struct SellMat {
 // Sets A.plane and A.halo from the offsets of a sample of rows
 // (stream-ordered, waits for the stream).
 void detect_plane(SellMat &A, hipStream_t s);
+// Largest grid side of the matrix-free stencil kernels (a wavefront holds
+// one x-line, two points per lane).
+constexpr int kStencilMaxN = 128;
+// Y[:, c] = A X[:, c] by the matrix-free stencil kernel when A is a
+// constant-coefficient 7-point operator it handles (false otherwise; the
+// same bits as the SELL kernels).
+bool spmm_stencil(const SellMat &A, const double *X, int64_t ldx, double *Y, int64_t ldy,
+                  int ncols, hipStream_t s);
+// The FP32 BlockPCG product w = A p over the listed 8-column tiles of its
+// interleaved blocks, matrix-free for a stencil operator (unshifted), with
+// partial p.w sums: stencil_pw_parts(A) per column (0 when A is not a
+// stencil operator), at partial[(tile 8 + t) parts + part].  The values
+// are the FP32 copies of the stencil's, as the matrix's FP32 copy holds.
+int stencil_pw_parts(const SellMat &A);
+void stencil_pw(const SellMat &A, const int *tiles, const int *ntact_p, int nt_bound,
+                const float *P, float *W, float *partial, hipStream_t s);
 // Y[:, c] = A X[:, c], c < ncols.
 void spmm(const SellMat &A, const double *X, int64_t ldx, double *Y, int64_t ldy,
           int ncols, hipStream_t s);

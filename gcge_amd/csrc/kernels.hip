@@ -660,6 +660,7 @@ void detect_plane(SellMat &A, hipStream_t s)
 void spmm(const SellMat &A, const double *X, int64_t ldx, double *Y, int64_t ldy, int ncols,
           hipStream_t s)
 {
+    if (spmm_stencil(A, X, ldx, Y, ldy, ncols, s)) return;
     launch_spmm<double>(A, A.val, X, ldx, Y, ldy, ncols, s);
 }
 

@@ -875,8 +875,14 @@ void pcg32_solve(void *matA, void **mv_b, void **mv_x, int *start, int *end, OPS
     const unsigned nb = blocks_for(n, 1024);
     // algorithmic bytes of one shifted SpMM over the active tiles: the CCS
     // arrays of A (and of B when shifted) once, p read once, w written once
-    const double mat_bytes = (double)A->sell.nnz_true * 8.0 + 4.0 * (double)(n + 1) +
-                             (sigma != 0.f ? (double)Bs->nnz_true * 8.0 : 0.0);
+    // A constant-coefficient 7-point A without shift is applied matrix-free
+    // (stencil.hip), with its own count of p.w partial sums per column
+    const int sparts = sigma == 0.f ? stencil_pw_parts(A->sell) : 0;
+    const int nparts_pw = sparts > 0 ? sparts : (int)nblk_row;
+    const double mat_bytes =
+        sparts > 0 ? 0.0
+                   : (double)A->sell.nnz_true * 8.0 + 4.0 * (double)(n + 1) +
+                         (sigma != 0.f ? (double)Bs->nnz_true * 8.0 : 0.0);
     int it = 0;
     // algorithmic bytes and flops of the vector updates over the active
     // tiles (as of the host's last read of the count, which can only
@@ -887,7 +893,11 @@ void pcg32_solve(void *matA, void **mv_b, void **mv_x, int *start, int *end, OPS
     // writes p; d += alpha p, r -= alpha w reads p, w, d, r and writes d, r.
     const double tile_vec = 4.0 * kC * (double)n;
     const float *vB = sigma != 0.f ? Bs->val32 : nullptr;
-    auto spmm_pw = [&](bool x, const int *tl, const int *nt_p) {
+    auto spmm_pw = [&](bool x, const int *tl, const int *nt_p, int nt_bound) {
+        if (sparts > 0) {
+            stencil_pw(A->sell, tl, nt_p, nt_bound, S.p, S.w, S.partial, st);
+            return;
+        }
 #define SPMM_PW(W, TWO, XC)                                                                   \
     k_spmm_pw<W, TWO, XC><<<nblk_row, kWG, 0, st>>>(n, A->sell.slice_ptr, A->sell.col,        \
                                                     A->sell.val32, Bs->col, vB, sigma, tl,     \
@@ -907,12 +917,12 @@ void pcg32_solve(void *matA, void **mv_b, void **mv_x, int *start, int *end, OPS
         }
 #undef SPMM_PW
     };
-    if (tune_pw && ntact > 0) {
+    if (tune_pw && ntact > 0 && sparts == 0) {
         // the tile list and count are set; p is written by the first
         // iteration before it is read, and w and the partials are
         // overwritten by every launch, so the timing runs leave no trace
         const int best =
-            tune_run(pkey, 2, 0, [&](int i) { spmm_pw(i == 1, S.tiles, S.ntact); }, st);
+            tune_run(pkey, 2, 0, [&](int i) { spmm_pw(i == 1, S.tiles, S.ntact, ntact); }, st);
         xc = best == 1;
     }
     // three launches per iteration up to kFuseMaxRows rows, with the state
@@ -950,18 +960,18 @@ void pcg32_solve(void *matA, void **mv_b, void **mv_x, int *start, int *end, OPS
         hipEvent_t ev = prof ? c.prof.begin(PROF_SPMM32, mat_bytes + 2.0 * 4.0 * kC * (double)nt * n,
                                             2.0 * (double)A->sell.nnz_true * kC * nt, st)
                              : nullptr;
-        spmm_pw(xc, tl, ntp);
+        spmm_pw(xc, tl, ntp, nt);
         if (prof) c.prof.end(ev, st);
         hipEvent_t evx = prof ? c.prof.begin(PROF_PCGVEC, (fused ? 6.0 : 3.0) * tile_vec * nt,
                                              (fused ? 6.0 : 4.0) * kC * (double)n * nt, st)
                               : nullptr;
         if (fused) {
             k_update_xr_pw<<<dim3(nb, nt), kWG, 0, st>>>(
-                n, tl, ntp, (int)nblk_row, S.partial, rho_buf[par], act_buf[par], S.p, S.w, S.x,
+                n, tl, ntp, nparts_pw, S.partial, rho_buf[par], act_buf[par], S.p, S.w, S.x,
                 S.r, S.partial_rr);
             if (prof) c.prof.end(evx, st);
         } else {
-            k_finish_pw<<<nt * kC, 64, 0, st>>>(S.tiles, S.ntact, (int)nblk_row, S.partial,
+            k_finish_pw<<<nt * kC, 64, 0, st>>>(S.tiles, S.ntact, nparts_pw, S.partial,
                                                 S.rho, S.active, S.alpha);
             k_update_xr<<<dim3(nb, nt), kWG, 0, st>>>(n, S.tiles, S.ntact, S.alpha, S.w, S.r,
                                                       S.partial);

@@ -37,9 +37,11 @@ class SolveResult:
 
 
 class DeviceMatrix:
-    """A symmetric CCS matrix uploaded to the device (HIPCCS_Create)."""
+    """A symmetric CCS matrix uploaded to the device (HIPCCS_Create).
+    general=True keeps a constant-coefficient 7-point grid operator on the
+    general sparse kernels instead of the matrix-free stencil ones."""
 
-    def __init__(self, A, fp32: bool = False):
+    def __init__(self, A, fp32: bool = False, general: bool = False):
         lib = abi.load()
         A = A.tocsc()
         self.n = A.shape[0]
@@ -49,14 +51,16 @@ class DeviceMatrix:
         m = abi.CCSMat(self._data.ctypes.data_as(abi.c_dbl_p),
                        self._indices.ctypes.data_as(abi.c_int_p),
                        self._indptr.ctypes.data_as(abi.c_int_p), self.n, self.n)
-        h = lib.HIPCCS_Create(C.byref(m), abi.GCGE_HIP_MAT_FP32 if fp32 else 0)
+        h = lib.HIPCCS_Create(C.byref(m), (abi.GCGE_HIP_MAT_FP32 if fp32 else 0) |
+                              (abi.GCGE_HIP_MAT_GENERAL if general else 0))
         if not h:
             raise ValueError("HIPCCS_Create rejected the matrix")
         self.handle = C.c_void_p(h)
 
     @classmethod
     def grid(cls, kind: int, dim: int, points: int, n: int, coef=None,
-             delta_rel: float = 0.0, fp32: bool = False) -> "DeviceMatrix":
+             delta_rel: float = 0.0, fp32: bool = False,
+             general: bool = False) -> "DeviceMatrix":
         """A grid operator assembled on the device (HIPCCS_CreateGrid):
         the same handle HIPCCS_Create gives for the host CCS of the
         operator, from the grid size and the node coefficients only."""
@@ -66,7 +70,8 @@ class DeviceMatrix:
             c = np.ascontiguousarray(coef, dtype=np.float64)
         op = abi.GridOp(kind, dim, points, n,
                         c.ctypes.data_as(abi.c_dbl_p) if c is not None else None, delta_rel)
-        h = lib.HIPCCS_CreateGrid(C.byref(op), abi.GCGE_HIP_MAT_FP32 if fp32 else 0)
+        h = lib.HIPCCS_CreateGrid(C.byref(op), (abi.GCGE_HIP_MAT_FP32 if fp32 else 0) |
+                                  (abi.GCGE_HIP_MAT_GENERAL if general else 0))
         if not h:
             raise ValueError("HIPCCS_CreateGrid rejected the operator")
         self = cls.__new__(cls)
@@ -77,6 +82,10 @@ class DeviceMatrix:
     def equal(self, other: "DeviceMatrix") -> bool:
         """True when both handles hold the same device arrays."""
         return bool(abi.load().HIPCCS_Equal(self.handle, other.handle))
+
+    def stencil(self) -> int:
+        """Grid side n when products run matrix-free as a 7-point stencil, else 0."""
+        return int(abi.load().HIPCCS_Stencil(self.handle))
 
     def info(self):
         lib = abi.load()

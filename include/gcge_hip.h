@@ -72,12 +72,22 @@ int gcge_host_GetOptionFromCommandLine(const char *name, char type, void *value,
  * app/app_ccs.c:78-82).  A single-precision copy of the values is kept
  * when flags & GCGE_HIP_MAT_FP32.  Returns NULL on invalid input. */
 #define GCGE_HIP_MAT_FP32 1
+/* With flags & GCGE_HIP_MAT_GENERAL the matrix is always multiplied as a
+ * general sparse matrix.  Without it, a constant-coefficient 7-point
+ * operator on an n^3 grid (n <= 128, point x + n y + n^2 z, one diagonal
+ * value and one neighbour value, as the stencil problems' A and B) is
+ * recognised and multiplied matrix-free, with the same results bit for
+ * bit. */
+#define GCGE_HIP_MAT_GENERAL 2
 void *HIPCCS_Create(const gcge_ccs_mat *host, int flags);
 void HIPCCS_Destroy(void **mat);
 /* Number of rows, stored nonzeros (including SELL padding) and device
  * bytes of a matrix handle. */
 void HIPCCS_Info(void *mat, int64_t *nrows, int64_t *nnz, int64_t *padded,
                  int64_t *bytes);
+/* The grid side n when the matrix is multiplied matrix-free as a
+ * constant-coefficient 7-point operator (see GCGE_HIP_MAT_GENERAL), else 0. */
+int HIPCCS_Stencil(void *mat);
 
 /* Grid operators assembled on the device, without a host CCS.  The
  * reference builds its matrices on the host and hands over a CCSMAT
@@ -100,7 +110,9 @@ void HIPCCS_Info(void *mat, int64_t *nrows, int64_t *nnz, int64_t *padded,
  *             such diagonal.
  *   DIAGONAL  (any dim, points ignored): diag(coef).
  * coef is a host array of N doubles (DIFFUSION, DIAGONAL), else unused.
- * A single-precision copy is kept when flags & GCGE_HIP_MAT_FP32.
+ * A single-precision copy is kept when flags & GCGE_HIP_MAT_FP32; the 3D
+ * LAPLACE and MASS operators are multiplied matrix-free unless flags &
+ * GCGE_HIP_MAT_GENERAL (as for HIPCCS_Create).
  * Returns NULL on an unsupported combination or N >= 2^31. */
 #define GCGE_GRID_LAPLACE 0
 #define GCGE_GRID_MASS 1

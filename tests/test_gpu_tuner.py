@@ -63,7 +63,7 @@ def test_spmm_shapes_give_identical_bits(lib, kind):
         A, W, ncand = P.config3_grid(18).A.to_ccs(), 32, 6
     n = A.shape[0]
     table = HipTable()
-    dA = DeviceMatrix(A)
+    dA = DeviceMatrix(A, general=True)  # the SELL shapes, not the matrix-free stencil
     lg = int(np.floor(np.log2(n)))
     for k in (1, 5, 24):
         X = np.random.default_rng(k).standard_normal((n, k))
