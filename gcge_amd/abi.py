@@ -108,7 +108,8 @@ class Result(C.Structure):
     _fields_ = [("status", C.c_int), ("nev_conv", C.c_int), ("num_iter", C.c_int),
                 ("concurrent", C.c_int), ("queue_seconds", C.c_double),
                 ("setup_seconds", C.c_double), ("solve_seconds", C.c_double),
-                ("total_seconds", C.c_double), ("eval", c_dbl_p), ("evec", c_dbl_p)]
+                ("total_seconds", C.c_double), ("eval", c_dbl_p), ("evec", c_dbl_p),
+                ("pool_bytes", C.c_int64)]
 
 
 class EpsOptions(C.Structure):
@@ -153,6 +154,8 @@ EXPORTS = {
                                   c_dbl_p, c_dbl_p]),
     "HIPCCS_Equal": (C.c_int, [C.c_void_p, C.c_void_p]),
     "HIPCCS_Stencil": (C.c_int, [C.c_void_p]),
+    "gcge_hip_set_memory_pool": (None, [C.c_int]),
+    "gcge_hip_memory_pool_stats": (None, [C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
     "HIPCCS_Info": (None, [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64),
                            C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
     "HIP_MultiVecSetFromHost": (None, [C.c_void_p, C.c_int, C.c_int, c_dbl_p, C.c_int64]),

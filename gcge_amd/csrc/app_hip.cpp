@@ -14,8 +14,11 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <iterator>
+#include <map>
 #include <mutex>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "app_hip_internal.h"
@@ -80,6 +83,131 @@ Ctx &ctx()
     return *t_ctx;
 }
 
+// ------------------------------------------------------------- memory pool
+struct MemPool {
+    bool on = false;
+    std::multimap<size_t, void *> cached_dev, cached_host;
+    size_t cached_bytes = 0;
+};
+
+namespace {
+// Cached bytes a pool keeps at most; past it the largest cached blocks go
+// back to HIP.
+constexpr size_t kPoolCap = (size_t)16 << 30;
+struct Block {
+    MemPool *pool;
+    size_t size;
+    bool host;
+};
+std::mutex g_blk_mu;                         // g_blocks and every pool's lists
+std::unordered_map<void *, Block> g_blocks;  // blocks allocated under a pool
+
+void *raw_alloc(size_t bytes, bool host)
+{
+    void *p = nullptr;
+    if (host)
+        HIPCHECK(hipHostMalloc(&p, bytes, hipHostMallocDefault));
+    else
+        HIPCHECK(hipMalloc(&p, bytes));
+    return p;
+}
+
+void raw_free(void *p, bool host)
+{
+    if (host)
+        HIPCHECK(hipHostFree(p));
+    else
+        HIPCHECK(hipFree(p));
+}
+
+void *pool_alloc(size_t bytes, bool host)
+{
+    if (bytes == 0) bytes = 1;
+    Ctx &c = ctx();
+    MemPool *pool = c.pool;
+    if (pool == nullptr || !pool->on) return raw_alloc(bytes, host);
+    {
+        std::lock_guard<std::mutex> lk(g_blk_mu);
+        auto &lst = host ? pool->cached_host : pool->cached_dev;
+        auto it = lst.lower_bound(bytes);
+        if (it != lst.end() && it->first <= 2 * bytes + 65536) {
+            void *p = it->second;
+            pool->cached_bytes -= it->first;
+            lst.erase(it);
+            return p;
+        }
+    }
+    const size_t size = (bytes + 255) / 256 * 256;
+    void *p = raw_alloc(size, host);
+    std::lock_guard<std::mutex> lk(g_blk_mu);
+    g_blocks[p] = Block{pool, size, host};
+    return p;
+}
+
+void pool_free(void *p, bool host)
+{
+    if (p == nullptr) return;
+    std::vector<std::pair<void *, bool>> release;
+    {
+        std::lock_guard<std::mutex> lk(g_blk_mu);
+        auto it = g_blocks.find(p);
+        if (it != g_blocks.end()) {
+            const Block b = it->second;
+            if (b.pool != nullptr && b.pool->on) {
+                (b.host ? b.pool->cached_host : b.pool->cached_dev).insert({b.size, p});
+                b.pool->cached_bytes += b.size;
+                // over the cap: hand the largest cached blocks back
+                while (b.pool->cached_bytes > kPoolCap) {
+                    auto &ld = b.pool->cached_dev, &lh = b.pool->cached_host;
+                    const bool from_dev =
+                        !ld.empty() && (lh.empty() || ld.rbegin()->first >= lh.rbegin()->first);
+                    auto &lst = from_dev ? ld : lh;
+                    auto last = std::prev(lst.end());
+                    b.pool->cached_bytes -= last->first;
+                    release.push_back({last->second, !from_dev});
+                    g_blocks.erase(last->second);
+                    lst.erase(last);
+                }
+                if (release.empty()) return;
+                p = nullptr;
+            } else {
+                g_blocks.erase(it);
+            }
+        }
+    }
+    for (auto &r : release) raw_free(r.first, r.second);
+    if (p != nullptr) raw_free(p, host);
+}
+
+// Hands a pool's cached blocks back to HIP and turns it off; blocks still in
+// use are freed to HIP when they are freed.
+void pool_flush(MemPool *pool, bool forget)
+{
+    std::vector<std::pair<void *, bool>> release;
+    {
+        std::lock_guard<std::mutex> lk(g_blk_mu);
+        pool->on = false;
+        for (auto *lst : {&pool->cached_dev, &pool->cached_host}) {
+            for (auto &e : *lst) {
+                release.push_back({e.second, lst == &pool->cached_host});
+                g_blocks.erase(e.second);
+            }
+            lst->clear();
+        }
+        pool->cached_bytes = 0;
+        if (forget)
+            for (auto &e : g_blocks)
+                if (e.second.pool == pool) e.second.pool = nullptr;
+    }
+    for (auto &r : release) raw_free(r.first, r.second);
+}
+}  // namespace
+
+void *dev_alloc_bytes(size_t bytes) { return pool_alloc(bytes, false); }
+void *pinned_alloc_bytes(size_t bytes) { return pool_alloc(bytes, true); }
+void dev_free(void *p) { pool_free(p, false); }
+void pinned_free(void *p) { pool_free(p, true); }
+
 uint64_t slice_hash(const std::vector<int64_t> &sp)
 {
     uint64_t h = 1469598103934665603ull;  // FNV-1a over the slice pointers
@@ -104,15 +232,19 @@ void release_ctx()
     c->sync();
     c->prof.collect();
     if (c->dev_eig) release_dev_eig(c->dev_eig);
-    if (c->d_scratch) HIPCHECK(hipFree(c->d_scratch));
-    if (c->d_small) HIPCHECK(hipFree(c->d_small));
-    if (c->h_pinned) HIPCHECK(hipHostFree(c->h_pinned));
+    if (c->d_scratch) dev_free(c->d_scratch);
+    if (c->d_small) dev_free(c->d_small);
+    if (c->h_pinned) pinned_free(c->h_pinned);
     for (Ctx::Staging &g : c->stage) {
-        if (g.h) HIPCHECK(hipHostFree(g.h));
-        if (g.d) HIPCHECK(hipFree(g.d));
+        if (g.h) pinned_free(g.h);
+        if (g.d) dev_free(g.d);
         if (g.ev) HIPCHECK(hipEventDestroy(g.ev));
     }
     for (hipEvent_t e : c->prof.pool) HIPCHECK(hipEventDestroy(e));
+    if (c->pool) {
+        pool_flush(c->pool, true);
+        delete c->pool;
+    }
     HIPCHECK(hipStreamDestroy(c->stream));
     delete c;
     t_ctx = nullptr;
@@ -124,9 +256,9 @@ double *Ctx::scratch(int64_t ndouble)
 {
     if (ndouble > scratch_cap) {
         sync();
-        if (d_scratch) HIPCHECK(hipFree(d_scratch));
+        if (d_scratch) dev_free(d_scratch);
         int64_t cap = ndouble + ndouble / 4 + 1024;
-        HIPCHECK(hipMalloc(reinterpret_cast<void **>(&d_scratch), cap * sizeof(double)));
+        dev_alloc(&d_scratch, cap * sizeof(double));
         scratch_cap = cap;
     }
     return d_scratch;
@@ -136,9 +268,9 @@ double *Ctx::small(int64_t ndouble)
 {
     if (ndouble > small_cap) {
         sync();
-        if (d_small) HIPCHECK(hipFree(d_small));
+        if (d_small) dev_free(d_small);
         int64_t cap = ndouble + 4096;
-        HIPCHECK(hipMalloc(reinterpret_cast<void **>(&d_small), cap * sizeof(double)));
+        dev_alloc(&d_small, cap * sizeof(double));
         small_cap = cap;
     }
     return d_small;
@@ -149,10 +281,9 @@ double *Ctx::pinned(int64_t ndouble)
     if (ndouble > pinned_cap) {
         // the stream may still read the old staging buffer
         HIPCHECK(hipStreamSynchronize(stream));
-        if (h_pinned) HIPCHECK(hipHostFree(h_pinned));
+        if (h_pinned) pinned_free(h_pinned);
         int64_t cap = ndouble + 4096;
-        HIPCHECK(hipHostMalloc(reinterpret_cast<void **>(&h_pinned), cap * sizeof(double),
-                               hipHostMallocDefault));
+        pinned_alloc(&h_pinned, cap * sizeof(double));
         pinned_cap = cap;
     }
     return h_pinned;
@@ -169,12 +300,11 @@ Ctx::Staging &Ctx::staging(int64_t ndouble)
         g.pending = false;
     }
     if (ndouble > g.cap) {
-        if (g.h) HIPCHECK(hipHostFree(g.h));
-        if (g.d) HIPCHECK(hipFree(g.d));
+        if (g.h) pinned_free(g.h);
+        if (g.d) dev_free(g.d);
         g.cap = ndouble + 4096;
-        HIPCHECK(hipHostMalloc(reinterpret_cast<void **>(&g.h), g.cap * sizeof(double),
-                               hipHostMallocDefault));
-        HIPCHECK(hipMalloc(reinterpret_cast<void **>(&g.d), g.cap * sizeof(double)));
+        pinned_alloc(&g.h, g.cap * sizeof(double));
+        dev_alloc(&g.d, g.cap * sizeof(double));
     }
     if (g.ev == nullptr) HIPCHECK(hipEventCreateWithFlags(&g.ev, hipEventDisableTiming));
     return g;
@@ -274,7 +404,7 @@ MV *mv_create(int64_t nrows, int64_t ncols)
     m->ldd = nrows;
     m->owner = true;
     const size_t bytes = (size_t)(nrows * (ncols > 0 ? ncols : 1)) * sizeof(double);
-    HIPCHECK(hipMalloc(reinterpret_cast<void **>(&m->d), bytes));
+    dev_alloc(&m->d, bytes);
     HIPCHECK(hipMemsetAsync(m->d, 0, bytes, c.stream));
     // another thread's stream may use the multivector next
     c.sync();
@@ -286,7 +416,7 @@ void mv_destroy(MV *m)
     if (!m) return;
     if (m->owner && m->d) {
         ctx().sync();
-        HIPCHECK(hipFree(m->d));
+        dev_free(m->d);
     }
     m->magic = 0;
     delete m;
@@ -797,9 +927,9 @@ void *HIPCCS_Create(const gcge_ccs_mat *host, int flags)
             }
         }
     }
-    HIPCHECK(hipMalloc(reinterpret_cast<void **>(&S.slice_ptr), sizeof(int64_t) * sp.size()));
-    HIPCHECK(hipMalloc(reinterpret_cast<void **>(&S.col), sizeof(int) * colv.size()));
-    HIPCHECK(hipMalloc(reinterpret_cast<void **>(&S.val), sizeof(double) * valv.size()));
+    dev_alloc(&S.slice_ptr, sizeof(int64_t) * sp.size());
+    dev_alloc(&S.col, sizeof(int) * colv.size());
+    dev_alloc(&S.val, sizeof(double) * valv.size());
     HIPCHECK(hipMemcpy(S.slice_ptr, sp.data(), sizeof(int64_t) * sp.size(), hipMemcpyHostToDevice));
     HIPCHECK(hipMemcpy(S.col, colv.data(), sizeof(int) * colv.size(), hipMemcpyHostToDevice));
     HIPCHECK(hipMemcpy(S.val, valv.data(), sizeof(double) * valv.size(), hipMemcpyHostToDevice));
@@ -808,7 +938,7 @@ void *HIPCCS_Create(const gcge_ccs_mat *host, int flags)
     if (flags & GCGE_HIP_MAT_FP32) {
         std::vector<float> v32(valv.size());
         for (size_t i = 0; i < valv.size(); ++i) v32[i] = (float)valv[i];
-        HIPCHECK(hipMalloc(reinterpret_cast<void **>(&S.val32), sizeof(float) * v32.size()));
+        dev_alloc(&S.val32, sizeof(float) * v32.size());
         HIPCHECK(hipMemcpy(S.val32, v32.data(), sizeof(float) * v32.size(), hipMemcpyHostToDevice));
         M->bytes += (int64_t)(sizeof(float) * v32.size());
     }
@@ -823,10 +953,10 @@ void HIPCCS_Destroy(void **mat)
     if (mat == nullptr || *mat == nullptr) return;
     Mat *M = mat_of(*mat, "HIPCCS_Destroy");
     ctx().sync();
-    HIPCHECK(hipFree(M->sell.slice_ptr));
-    HIPCHECK(hipFree(M->sell.col));
-    HIPCHECK(hipFree(M->sell.val));
-    if (M->sell.val32) HIPCHECK(hipFree(M->sell.val32));
+    dev_free(M->sell.slice_ptr);
+    dev_free(M->sell.col);
+    dev_free(M->sell.val);
+    if (M->sell.val32) dev_free(M->sell.val32);
     M->magic = 0;
     delete M;
     *mat = nullptr;
@@ -896,6 +1026,30 @@ int gcge_hip_last_error(char *buf, int len)
 }
 
 int gcge_hip_device(void) { return ctx().device; }
+
+void gcge_hip_set_memory_pool(int on)
+{
+    Ctx &c = ctx();
+    if (on) {
+        if (c.pool == nullptr) c.pool = new MemPool();
+        std::lock_guard<std::mutex> lk(g_blk_mu);
+        c.pool->on = true;
+    } else if (c.pool != nullptr) {
+        c.sync();
+        pool_flush(c.pool, false);
+    }
+}
+
+void gcge_hip_memory_pool_stats(int64_t *allocated, int64_t *cached)
+{
+    Ctx &c = ctx();
+    int64_t a = 0;
+    std::lock_guard<std::mutex> lk(g_blk_mu);
+    for (auto &e : g_blocks)
+        if (e.second.pool == c.pool && c.pool != nullptr) a += (int64_t)e.second.size;
+    if (allocated) *allocated = a;
+    if (cached) *cached = c.pool ? (int64_t)c.pool->cached_bytes : 0;
+}
 
 void gcge_hip_profile_enable(int on)
 {

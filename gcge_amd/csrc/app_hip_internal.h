@@ -18,6 +18,29 @@ void record_error(const char *what);
 void hip_check(hipError_t e, const char *expr, const char *file, int line);
 #define HIPCHECK(x) ::gcge::hip_check((x), #x, __FILE__, __LINE__)
 
+// Device and pinned host memory of the library.  A thread may keep a pool
+// (gcge_hip_set_memory_pool): while it is on, blocks the thread allocated
+// are cached when freed, from any thread, and later allocations of the
+// thread reuse a cached block of at least the size asked (and at most about
+// twice it) before asking HIP, so a thread that runs request after request
+// stops allocating once its pool holds what a request needs.  Without a
+// pool these are hipMalloc / hipFree and hipHostMalloc / hipHostFree.
+void *dev_alloc_bytes(size_t bytes);
+void *pinned_alloc_bytes(size_t bytes);
+void dev_free(void *p);
+void pinned_free(void *p);
+template <typename T>
+void dev_alloc(T **p, size_t bytes)
+{
+    *p = static_cast<T *>(dev_alloc_bytes(bytes));
+}
+template <typename T>
+void pinned_alloc(T **p, size_t bytes)
+{
+    *p = static_cast<T *>(pinned_alloc_bytes(bytes));
+}
+struct MemPool;
+
 struct Stats {
     int64_t spmm_calls = 0, spmm_cols = 0, gram_calls = 0, lincomb_calls = 0, dot_calls = 0;
     double gram_flops = 0.0, lincomb_flops = 0.0;
@@ -65,6 +88,8 @@ struct Ctx {
     Profiler prof;
     // the device dense eigensolver's handle and buffers (dense_eig.cpp)
     void *dev_eig = nullptr;
+    // the thread's memory pool, or null (dev_alloc_bytes)
+    MemPool *pool = nullptr;
     // Upload staging for the coefficients of asynchronous slots: a ring of
     // pinned host / device buffer pairs, each reused only after the event
     // recorded behind the launches that read it has completed.

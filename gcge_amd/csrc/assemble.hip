@@ -276,12 +276,12 @@ void *HIPCCS_CreateGrid(const gcge_grid_op *op, int flags)
     S.nnz_true = nnz;
     S.structure_hash = slice_hash(sp);
 
-    HIPCHECK(hipMalloc(reinterpret_cast<void **>(&S.slice_ptr), sizeof(int64_t) * sp.size()));
-    HIPCHECK(hipMalloc(reinterpret_cast<void **>(&S.col), sizeof(int) * (size_t)S.nnz));
-    HIPCHECK(hipMalloc(reinterpret_cast<void **>(&S.val), sizeof(double) * (size_t)S.nnz));
+    dev_alloc(&S.slice_ptr, sizeof(int64_t) * sp.size());
+    dev_alloc(&S.col, sizeof(int) * (size_t)S.nnz);
+    dev_alloc(&S.val, sizeof(double) * (size_t)S.nnz);
     M->bytes = (int64_t)(sizeof(int64_t) * sp.size() + (sizeof(int) + sizeof(double)) * S.nnz);
     if (flags & GCGE_HIP_MAT_FP32) {
-        HIPCHECK(hipMalloc(reinterpret_cast<void **>(&S.val32), sizeof(float) * (size_t)S.nnz));
+        dev_alloc(&S.val32, sizeof(float) * (size_t)S.nnz);
         M->bytes += (int64_t)(sizeof(float) * S.nnz);
     }
     HIPCHECK(hipMemcpyAsync(S.slice_ptr, sp.data(), sizeof(int64_t) * sp.size(),
@@ -313,7 +313,7 @@ void *HIPCCS_CreateGrid(const gcge_grid_op *op, int flags)
     double *tmp = nullptr;
     if (op->coef != nullptr && (form == kDiff5 || form == kDiff27 || form == kDiag)) {
         const int64_t extra = form == kDiff27 ? N + nblk + 1 : 0;
-        HIPCHECK(hipMalloc(reinterpret_cast<void **>(&tmp), sizeof(double) * (size_t)(N + extra)));
+        dev_alloc(&tmp, sizeof(double) * (size_t)(N + extra));
         HIPCHECK(hipMemcpyAsync(tmp, op->coef, sizeof(double) * (size_t)N, hipMemcpyHostToDevice,
                                 st));
         a.coef = tmp;
@@ -336,7 +336,7 @@ void *HIPCCS_CreateGrid(const gcge_grid_op *op, int flags)
     }
     }
     HIPCHECK(hipStreamSynchronize(st));  // the host coefficients and sp may go now
-    if (tmp) HIPCHECK(hipFree(tmp));
+    if (tmp) dev_free(tmp);
     detect_plane(S, st);
     if (form == kConst7 && !(flags & GCGE_HIP_MAT_GENERAL) && n >= 4 && n <= kStencilMaxN) {
         S.stencil_n = (int)n;

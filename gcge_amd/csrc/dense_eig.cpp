@@ -173,13 +173,13 @@ struct DevEig {
 void free_buffers(DevEig &d)
 {
     if (d.dA == nullptr) return;
-    HIPCHECK(hipFree(d.dA));
-    HIPCHECK(hipFree(d.dW));
-    HIPCHECK(hipFree(d.dE));
-    HIPCHECK(hipFree(d.dinfo));
-    HIPCHECK(hipHostFree(d.hA));
-    HIPCHECK(hipHostFree(d.hW));
-    HIPCHECK(hipHostFree(d.hinfo));
+    dev_free(d.dA);
+    dev_free(d.dW);
+    dev_free(d.dE);
+    dev_free(d.dinfo);
+    pinned_free(d.hA);
+    pinned_free(d.hW);
+    pinned_free(d.hinfo);
     d.dA = d.dW = d.dE = d.hA = d.hW = nullptr;
     d.dinfo = d.hinfo = nullptr;
     d.cap = 0;
@@ -216,15 +216,13 @@ int eigh_device(int n, double *A, int lda, double *w, double *Z, int ldz)
         free_buffers(d);
         d.cap = n + 64;
         const size_t sq = sizeof(double) * (size_t)d.cap * d.cap;
-        HIPCHECK(hipMalloc(reinterpret_cast<void **>(&d.dA), sq));
-        HIPCHECK(hipMalloc(reinterpret_cast<void **>(&d.dW), sizeof(double) * d.cap));
-        HIPCHECK(hipMalloc(reinterpret_cast<void **>(&d.dE), sizeof(double) * d.cap));
-        HIPCHECK(hipMalloc(reinterpret_cast<void **>(&d.dinfo), sizeof(rocblas_int)));
-        HIPCHECK(hipHostMalloc(reinterpret_cast<void **>(&d.hA), sq, hipHostMallocDefault));
-        HIPCHECK(hipHostMalloc(reinterpret_cast<void **>(&d.hW), sizeof(double) * d.cap,
-                               hipHostMallocDefault));
-        HIPCHECK(hipHostMalloc(reinterpret_cast<void **>(&d.hinfo), sizeof(rocblas_int),
-                               hipHostMallocDefault));
+        dev_alloc(&d.dA, sq);
+        dev_alloc(&d.dW, sizeof(double) * d.cap);
+        dev_alloc(&d.dE, sizeof(double) * d.cap);
+        dev_alloc(&d.dinfo, sizeof(rocblas_int));
+        pinned_alloc(&d.hA, sq);
+        pinned_alloc(&d.hW, sizeof(double) * d.cap);
+        pinned_alloc(&d.hinfo, sizeof(rocblas_int));
     }
     const size_t col = sizeof(double) * (size_t)n;
     for (int j = 0; j < n; ++j) memcpy(d.hA + (size_t)n * j, A + (size_t)lda * j, col);

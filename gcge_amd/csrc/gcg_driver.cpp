@@ -151,14 +151,9 @@ void orth_self(void **x, int start_x, int *end_x, void *B, const OrthParams &p, 
 // the B inner product: block classical Gram-Schmidt passes until the
 // largest coefficient is below reorth_tol, at most 1 + max_reorth
 // (src/ops_orth.c:231-269), or at most max_passes when that is positive.
-// first_norms, if given, receives for each column of the block the norm
-// of its first-pass coefficients: the size of its component along the
-// earlier (B-orthonormal) columns before their removal.
 void remove_previous(void **x, int start_x, int end_x, void *B, const OrthParams &p,
-                     void **mv_ws, OPS *ops, int max_passes = 0,
-                     std::vector<double> *first_norms = nullptr)
+                     void **mv_ws, OPS *ops, int max_passes = 0)
 {
-    if (first_norms) first_norms->clear();
     if (start_x <= 0 || end_x <= start_x) return;
     int s[2], e[2];
     double one = 1.0;
@@ -170,14 +165,6 @@ void remove_previous(void **x, int start_x, int end_x, void *B, const OrthParams
     if (max_passes > 0) passes = std::min(passes, max_passes);
     for (int it = 0; it < passes; ++it) {
         ops->MultiVecQtAP('S', 'N', x, B, x, 0, s, e, coef.data(), e[0] - s[0], mv_ws, ops);
-        if (it == 0 && first_norms) {
-            const int ld = e[0] - s[0];
-            for (int j = 0; j < e[1] - s[1]; ++j) {
-                double q = 0.0;
-                for (int i = 0; i < ld; ++i) q += coef[(size_t)ld * j + i] * coef[(size_t)ld * j + i];
-                first_norms->push_back(std::sqrt(q));
-            }
-        }
         double amax = 0.0;
         for (int64_t i = 0; i < len; ++i) {
             coef[i] = -coef[i];
@@ -225,6 +212,17 @@ bool cholesky_upper(int n, const std::vector<double> &G, std::vector<double> &R,
 // of one B-product and two passes per column.  A column whose B-norm is
 // below zero_tol (which MGS drops) or a numerically dependent block sends
 // the block to the column-by-column MGS.
+//
+// The first pass scales the block's remainders along the earlier columns
+// (of order eps after the removal) up: by a column's cancellation when it
+// lost digits to the earlier columns, and by cond(R) when the block's own
+// columns are nearly dependent.  The second pass orthonormalizes only
+// within the block, so the earlier columns are removed once more between
+// the passes, as MGS reorthogonalizes.  Without it, blocks of the mixed
+// path meeting the degenerate clusters of 3D Laplacians lost
+// B-orthogonality: a 16^3 solve stalled on spurious near-zero Ritz values
+// and a 48^3 solve returned vectors with max|X^T B X - I| = 2e-8.  It costs
+// one Gram and one LinearComb per block (1.5 % of the 128^3 mixed solve).
 void cholqr(void **x, int start_x, int *end_x, void *B, const OrthParams &p, void **mv_ws,
             OPS *ops, int64_t *fallbacks)
 {
@@ -233,23 +231,11 @@ void cholqr(void **x, int start_x, int *end_x, void *B, const OrthParams &p, voi
     // second pass leaves components of order eps * ||x||, Giraud, Langou
     // and Rozloznik 2005); the reference's absolute test on the largest
     // coefficient (50 eps) would run a third pass on unit-size columns
-    std::vector<double> along;
-    remove_previous(x, start_x, *end_x, B, p, mv_ws, ops, 2, &along);
+    remove_previous(x, start_x, *end_x, B, p, mv_ws, ops, 2);
     const int n = *end_x - start_x;
     std::vector<double> G((size_t)n * n), R, Rinv((size_t)n * n), dinv((size_t)n);
-    bool again = false;
     for (int pass = 0; pass < 2; ++pass) {
-        // A column that was nearly in the span of the earlier columns keeps
-        // a remainder of order eps times its old component along them;
-        // normalizing it (the first pass) scales that remainder up by the
-        // cancellation.  When some column lost more than four digits
-        // (component along the earlier columns over 1e4 times what is
-        // left), the earlier columns are removed once more before the
-        // second pass, as MGS reorthogonalizes a column whose norm dropped.
-        // Without it, a block meeting a converged degenerate cluster lost
-        // B-orthogonality and the Rayleigh-Ritz step returned spurious
-        // near-zero eigenvalues.
-        if (pass == 1 && again) remove_previous(x, start_x, *end_x, B, p, mv_ws, ops, 1);
+        if (pass == 1) remove_previous(x, start_x, *end_x, B, p, mv_ws, ops, 1);
         int s[2] = {start_x, start_x}, e[2] = {*end_x, *end_x};
         ops->MultiVecQtAP('S', 'S', x, B, x, 0, s, e, G.data(), n, mv_ws, ops);
         bool ok = true;
@@ -257,8 +243,6 @@ void cholqr(void **x, int start_x, int *end_x, void *B, const OrthParams &p, voi
             const double d = std::sqrt(std::max(G[(size_t)n * i + i], 0.0));
             ok = d >= p.zero_tol;
             dinv[i] = ok ? 1.0 / d : 0.0;
-            if (pass == 0 && ok && (size_t)i < along.size() && along[(size_t)i] > 1e4 * d)
-                again = true;
         }
         if (ok) {
             for (int j = 0; j < n; ++j)
@@ -701,6 +685,35 @@ void dense_eigh(Solver &S, int n, double *M, double *w, double *Z)
     }
 }
 
+// The returned values of the k converged pairs as their vectors' Rayleigh
+// quotients x^T A x / x^T B x, and the vectors scaled to unit B-norm (the
+// block-orthonormalization path).  Over many iterations the converged
+// (locked) vectors can drift from B-orthonormality, and a Ritz value of the
+// projected problem then differs from its vector's Rayleigh quotient by
+// about that drift (a served 48^3 solve reached max|X^T B X - I| = 2e-8
+// with Ritz values 2e-9 from their Rayleigh quotients, which were exact to
+// 2e-14).  The vectors keep their directions, so each residual
+// ||A x - theta B x|| only drops (theta is the quotient that minimizes it
+// in the B^{-1} norm).  V (no longer needed) is the workspace.
+void final_rayleigh_quotients(Solver &S, void **X, int k, double *eval, void **V)
+{
+    Timer t(S, T_RV);
+    OPS *ops = S.ops;
+    if (k <= 0) return;
+    std::vector<double> xa((size_t)k), xb((size_t)k), sc((size_t)k);
+    int s[2] = {0, 0}, e[2] = {k, k};
+    ops->MatDotMultiVec(S.B, X, V, s, e, ops);
+    ops->MultiVecInnerProd('D', X, V, 0, s, e, xb.data(), 1, ops);
+    ops->MatDotMultiVec(S.A, X, V, s, e, ops);
+    ops->MultiVecInnerProd('D', X, V, 0, s, e, xa.data(), 1, ops);
+    for (int j = 0; j < k; ++j) {
+        if (!(xb[j] > 0.0)) return;  // not a B-positive vector: keep the iteration's
+        sc[j] = 1.0 / std::sqrt(xb[j]);
+    }
+    for (int j = 0; j < k; ++j) eval[j] = xa[j] / xb[j];
+    ops->MultiVecLinearComb(nullptr, X, 0, s, e, nullptr, 0, sc.data(), 1, ops);
+}
+
 void rayleigh_ritz(Solver &S, int nev_conv, void **V)
 {
     Timer t(S, T_RR);
@@ -876,6 +889,7 @@ void gcg(void *A, void *B, double *eval, void **evec, int nevGiven, int *nevConv
     } while (numIter < numIterMax);
     S.numIter = numIter + (S.numIterMax - numIterMax);
     for (int i = 0; i < S.sizeX; ++i) eval[i] = S.ss_eval[i];
+    if (S.orth_method == GCGE_ORTH_CHOLQR) final_rayleigh_quotients(S, ritz, *nevConv, eval, V);
     if (S.printtime) {
         double tot = 0.0;
         for (int k : {T_INITX, T_CHECK, T_COMPP, T_RR, T_RV, T_COMPW, T_COMPX}) tot += S.times[k];

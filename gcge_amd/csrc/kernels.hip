@@ -628,13 +628,13 @@ void detect_plane(SellMat &A, hipStream_t s)
     if (A.nrows < 4096 || A.max_width > 32) return;
     constexpr int kSample = 1024;
     int64_t *d = nullptr;
-    HIPCHECK(hipMalloc(reinterpret_cast<void **>(&d), sizeof(int64_t) * kSample * 32));
+    dev_alloc(&d, sizeof(int64_t) * kSample * 32);
     k_row_offsets<<<(kSample + kWG - 1) / kWG, kWG, 0, s>>>(A.nrows, kSample, A.slice_ptr,
                                                             A.col, d);
     std::vector<int64_t> h((size_t)kSample * 32);
     HIPCHECK(hipMemcpyAsync(h.data(), d, sizeof(int64_t) * h.size(), hipMemcpyDeviceToHost, s));
     HIPCHECK(hipStreamSynchronize(s));
-    HIPCHECK(hipFree(d));
+    dev_free(d);
     // plane: median over rows of the largest offset
     std::vector<int64_t> m(kSample);
     for (int k = 0; k < kSample; ++k)

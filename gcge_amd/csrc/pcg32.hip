@@ -747,25 +747,22 @@ struct PCG32 {
         ncap = nc8;
         const size_t blkbytes = sizeof(float) * (size_t)n * ncap;
         for (float **q : {&x, &r, &p, &w})
-            HIPCHECK(hipMalloc(reinterpret_cast<void **>(q), blkbytes));
+            dev_alloc(q, blkbytes);
         for (float **q : {&rho, &rho_old, &init_res, &alpha})
-            HIPCHECK(hipMalloc(reinterpret_cast<void **>(q), sizeof(float) * ncap));
-        HIPCHECK(hipMalloc(reinterpret_cast<void **>(&partial),
-                           sizeof(float) * (size_t)ncap * nparts_max()));
-        HIPCHECK(hipMalloc(reinterpret_cast<void **>(&colsum), sizeof(double) * ncap));
-        HIPCHECK(hipMalloc(reinterpret_cast<void **>(&tiles), sizeof(int) * (ncap / kC)));
-        HIPCHECK(hipMalloc(reinterpret_cast<void **>(&active), sizeof(int) * ncap));
-        HIPCHECK(hipMalloc(reinterpret_cast<void **>(&ntact), sizeof(int)));
-        HIPCHECK(hipMalloc(reinterpret_cast<void **>(&dropped), sizeof(int) * (ncap / kC)));
-        HIPCHECK(hipMalloc(reinterpret_cast<void **>(&ndropped), sizeof(int)));
-        HIPCHECK(hipHostMalloc(reinterpret_cast<void **>(&h_ntact), sizeof(int),
-                               hipHostMallocDefault));
-        HIPCHECK(hipMalloc(reinterpret_cast<void **>(&rho_b), sizeof(float) * ncap));
-        HIPCHECK(hipMalloc(reinterpret_cast<void **>(&active_b), sizeof(int) * ncap));
-        HIPCHECK(hipMalloc(reinterpret_cast<void **>(&tiles_b), sizeof(int) * (ncap / kC)));
-        HIPCHECK(hipMalloc(reinterpret_cast<void **>(&ntact_b), sizeof(int)));
-        HIPCHECK(hipMalloc(reinterpret_cast<void **>(&partial_rr),
-                           sizeof(float) * (size_t)ncap * 1024));
+            dev_alloc(q, sizeof(float) * ncap);
+        dev_alloc(&partial, sizeof(float) * (size_t)ncap * nparts_max());
+        dev_alloc(&colsum, sizeof(double) * ncap);
+        dev_alloc(&tiles, sizeof(int) * (ncap / kC));
+        dev_alloc(&active, sizeof(int) * ncap);
+        dev_alloc(&ntact, sizeof(int));
+        dev_alloc(&dropped, sizeof(int) * (ncap / kC));
+        dev_alloc(&ndropped, sizeof(int));
+        pinned_alloc(&h_ntact, sizeof(int));
+        dev_alloc(&rho_b, sizeof(float) * ncap);
+        dev_alloc(&active_b, sizeof(int) * ncap);
+        dev_alloc(&tiles_b, sizeof(int) * (ncap / kC));
+        dev_alloc(&ntact_b, sizeof(int));
+        dev_alloc(&partial_rr, sizeof(float) * (size_t)ncap * 1024);
     }
     int64_t nparts_max() const { return (n + kWG - 1) / kWG; }
     void release()
@@ -773,19 +770,19 @@ struct PCG32 {
         drop_graph();
         if (!x) return;
         ctx().sync();
-        for (float *q : {x, r, p, w, rho, rho_old, init_res, alpha, partial}) HIPCHECK(hipFree(q));
-        HIPCHECK(hipFree(colsum));
-        HIPCHECK(hipFree(tiles));
-        HIPCHECK(hipFree(active));
-        HIPCHECK(hipFree(ntact));
-        HIPCHECK(hipFree(dropped));
-        HIPCHECK(hipFree(ndropped));
-        HIPCHECK(hipHostFree(h_ntact));
-        HIPCHECK(hipFree(rho_b));
-        HIPCHECK(hipFree(active_b));
-        HIPCHECK(hipFree(tiles_b));
-        HIPCHECK(hipFree(ntact_b));
-        HIPCHECK(hipFree(partial_rr));
+        for (float *q : {x, r, p, w, rho, rho_old, init_res, alpha, partial}) dev_free(q);
+        dev_free(colsum);
+        dev_free(tiles);
+        dev_free(active);
+        dev_free(ntact);
+        dev_free(dropped);
+        dev_free(ndropped);
+        pinned_free(h_ntact);
+        dev_free(rho_b);
+        dev_free(active_b);
+        dev_free(tiles_b);
+        dev_free(ntact_b);
+        dev_free(partial_rr);
         x = nullptr;
     }
 };

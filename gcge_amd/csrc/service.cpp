@@ -120,6 +120,7 @@ int run_one(Job &j)
     HIPCCS_Destroy(&A);
     if (B) HIPCCS_Destroy(&B);
     gcge_hip_synchronize();
+    gcge_hip_memory_pool_stats(&r.pool_bytes, nullptr);
     r.setup_seconds = t1 - t0;
     r.solve_seconds = t2 - t1;
     return 0;
@@ -127,6 +128,11 @@ int run_one(Job &j)
 
 void worker(Service *s)
 {
+    // device buffers freed by one request are kept for the next ones, so a
+    // running request does not allocate or free device memory once the
+    // worker's pool holds what its requests need (hipFree waits for the
+    // whole device, which would stall every other worker)
+    gcge_hip_set_memory_pool(1);
     for (;;) {
         Job *j = nullptr;
         {

@@ -27,6 +27,7 @@ class ServedResult:
     solve_seconds: float
     total_seconds: float
     evec: np.ndarray | None = None
+    pool_bytes: int = 0
 
 
 def _ccs(M):
@@ -102,7 +103,7 @@ class Service:
         k = res.nev_conv
         return ServedResult(ev[:k].copy(), k, res.num_iter, res.concurrent, res.queue_seconds,
                             res.setup_seconds, res.solve_seconds, res.total_seconds,
-                            vec[:, :k].copy() if vec is not None else None)
+                            vec[:, :k].copy() if vec is not None else None, res.pool_bytes)
 
     def stats(self):
         done, conc, used = C.c_int64(), C.c_int(), C.c_int64()

@@ -180,6 +180,18 @@ void gcge_hip_synchronize(void);
  * next call from the thread creates a new one.  Worker threads call it
  * before they exit. */
 void gcge_hip_thread_release(void);
+/* Memory pool of the calling thread (off by default): with on != 0,
+ * device and pinned host buffers the library allocates on this thread are
+ * cached when freed and reused by the thread's later allocations of about
+ * the same size (at least the size asked, at most about twice it) instead
+ * of going back to HIP, so a thread that serves request after request stops
+ * calling hipMalloc and hipFree (which waits for the device) once its pool
+ * holds what a request needs.  At most 16 GiB stays cached per thread; off
+ * (or gcge_hip_thread_release) hands the cached blocks back.  The solve
+ * service's workers run with it on.  Stats: bytes the thread's pool has
+ * allocated from HIP (in use or cached), and bytes cached. */
+void gcge_hip_set_memory_pool(int on);
+void gcge_hip_memory_pool_stats(int64_t *allocated, int64_t *cached);
 /* Copies the last error message (empty if none) into buf; returns its
  * code (0 = none). */
 int gcge_hip_last_error(char *buf, int len);
@@ -357,6 +369,9 @@ typedef struct {
     double queue_seconds, setup_seconds, solve_seconds, total_seconds;
     double *eval;   /* caller's array of nev_max doubles, or NULL */
     double *evec;   /* caller's N x nev_max column-major array, or NULL */
+    int64_t pool_bytes; /* device and pinned bytes the worker's memory pool
+                           holds from HIP after this request (in use or
+                           cached for the next requests) */
 } gcge_result;
 /* Device bytes a request holds while it runs.  A request with nev_given
  * outside [0, nev_init], or nev_given > 0 without x0, is refused

@@ -99,3 +99,23 @@ def test_warm_start_from_a_nearby_problem(hip, mode):
     check(p1, cold)
     check(p1, warm)
     assert warm.num_iter < cold.num_iter, (warm.num_iter, cold.num_iter)
+
+
+@pytest.mark.parametrize("mode", ["mixed", "fp64"])
+def test_worker_pool_stops_growing(hip, mode):
+    """A worker's memory pool soon holds what its requests need: after a
+    first pass over three requests, further passes allocate nothing more
+    from HIP (pool bytes stay put), and the results stay the same bit for
+    bit."""
+    probs = PROBS[:3]
+    passes = []
+    with Service(workers=1) as svc:
+        for _ in range(3):
+            passes.append([svc.wait(svc.submit(p, mode=mode, seed=5)) for p in probs])
+    sizes = [[r.pool_bytes for r in ps] for ps in passes]
+    print(f"# pool bytes per request, three passes: {sizes}")
+    assert sizes[0][-1] > 0
+    assert sizes[1] == sizes[2] == [sizes[1][-1]] * 3
+    for ps in passes[1:]:
+        for a, b in zip(passes[0], ps):
+            assert np.array_equal(a.eval, b.eval) and a.num_iter == b.num_iter

@@ -256,3 +256,67 @@ def test_info(mats, hip):
     info = dM.info()
     assert info["nrows"] == M.shape[0] and info["nnz"] == M.nnz
     assert info["stored"] >= M.nnz and info["stored"] % 64 == 0
+
+
+@pytest.fixture(scope="module")
+def tall(hip):
+    """A matrix handle of 300,000 rows, for multivectors past the row count
+    (262,144) where LinearComb's operand ring is 5 steps deep."""
+    return DeviceMatrix(sp.identity(300_000, format="csc"))
+
+
+@pytest.mark.parametrize("mode", ["overwrite", "scalar", "percol"])
+def test_linear_comb_tall(table, tall, mode):
+    """Y = X C + Y diag(beta) at 300,000 rows, 230 -> 130 columns (two
+    64-column output tiles plus a partial one), every beta mode."""
+    n, m, k = 300_000, 230, 130
+    rng = np.random.default_rng(12)
+    X = rng.standard_normal((n, m + 1))
+    Y0 = rng.standard_normal((n, k + 2))
+    ldc = m + 1
+    coef = np.asfortranarray(rng.standard_normal((ldc, k)))
+    beta_arr = rng.standard_normal(k)
+    hx, hy = upload(table, X, tall), upload(table, Y0, tall)
+    fn = table.slot("MultiVecLinearComb")
+    s, e = ia(1, 2), ia(1 + m, 2 + k)
+    cptr = coef.ctypes.data_as(abi.c_dbl_p)
+    ref = Y0.copy()
+    if mode == "overwrite":
+        fn(hx, hy, 0, s, e, cptr, ldc, None, 0, table.ops)
+        O.linear_comb(X[:, 1:1 + m], ref[:, 2:2 + k], coef[:m], None)
+    elif mode == "scalar":
+        b = C.c_double(-0.5)
+        fn(hx, hy, 0, s, e, cptr, ldc, C.byref(b), 0, table.ops)
+        O.linear_comb(X[:, 1:1 + m], ref[:, 2:2 + k], coef[:m], -0.5)
+    else:
+        fn(hx, hy, 0, s, e, cptr, ldc, beta_arr.ctypes.data_as(abi.c_dbl_p), 1, table.ops)
+        O.linear_comb(X[:, 1:1 + m], ref[:, 2:2 + k], coef[:m], beta_arr)
+    Y = table.download(hy, 0, k + 2, n)
+    table.destroy_multivec(hx, m + 1)
+    table.destroy_multivec(hy, k + 2)
+    close(Y, ref)
+
+
+@pytest.mark.parametrize("nsd", ["N", "S"])
+def test_inner_prod_tall(table, tall, nsd):
+    """Matrix-core Gram at 300,000 rows: 160 x 130 ('N') and 130 x 130
+    ('S'), with ldIP larger than the block."""
+    n = 300_000
+    m, k = (160, 130) if nsd == "N" else (130, 130)
+    rng = np.random.default_rng(13)
+    X = rng.standard_normal((n, m + 2))
+    Y = X.copy() if nsd == "S" else rng.standard_normal((n, k))
+    hx = upload(table, X, tall)
+    hy = hx if nsd == "S" else upload(table, Y, tall)
+    ld = m + 3
+    ip = np.full(ld * k, 99.0)
+    sy = 2 if nsd == "S" else 0
+    table.slot("MultiVecInnerProd")(nsd.encode(), hx, hy, 0, ia(2, sy), ia(2 + m, sy + k),
+                                    ip.ctypes.data_as(abi.c_dbl_p), ld, table.ops)
+    G = O.inner_prod(nsd, X[:, 2:2 + m], (X if nsd == "S" else Y)[:, sy:sy + k])
+    got = ip.reshape((ld, k), order="F")
+    close(got[:m], G)
+    assert np.all(got[m:] == 99.0)
+    table.destroy_multivec(hx, m + 2)
+    if nsd == "N":
+        table.destroy_multivec(hy, k)
