@@ -371,7 +371,8 @@ def served(args):
     log(f"{args.config}: generated {len(items)} problems in {time.perf_counter() - t0:.1f}s")
     check_every = max(1, len(items) // 8)
     res = [None] * len(items)
-    with Service(workers=args.workers, hbm_budget=args.hbm_budget) as svc:
+    with Service(workers=args.workers, hbm_budget=args.hbm_budget,
+                 batching=args.batching == "on") as svc:
         # warm the workers' kernels and handles on a small problem each
         warm = [svc.submit(P.stencil_problem(24, 2, 8), seed=1) for _ in range(args.workers)]
         for t in warm:
@@ -388,6 +389,7 @@ def served(args):
             res[k] = svc.wait(t)
         wall = time.perf_counter() - start
         st = svc.stats()
+        bst = svc.batch_stats()
     lat = np.array([r.total_seconds for r in res])
     conv = sum(r.nev_conv for r in res)
     short = sum(1 for (_, p), r in zip(items, res) if r.nev_conv < p.nev)
@@ -405,7 +407,9 @@ def served(args):
         "ms_per_step": 1e3 * wall, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f64+f32", "data": "synthetic",
         "config": {"workload": what, "path": "mixed", "workers": args.workers,
+                   "batched_inner_solves": args.batching == "on",
                    "hbm_budget_bytes": args.hbm_budget, "parallelism": "single GPU"},
+        "batched_rounds": bst,
         "latency_ms": {"p50": 1e3 * float(np.percentile(lat, 50)),
                        "p99": 1e3 * float(np.percentile(lat, 99))},
         "requests": len(items), "eigenpairs_converged": conv,
@@ -432,7 +436,9 @@ def main():
     ap.add_argument("--copy-probe", action="store_true")
     ap.add_argument("--count", type=int, default=512, help="cfg4/cfg5 requests")
     ap.add_argument("--qps", type=float, default=20.0, help="cfg5 arrival rate")
-    ap.add_argument("--workers", type=int, default=8, help="cfg4/cfg5 service workers")
+    ap.add_argument("--workers", type=int, default=16, help="cfg4/cfg5 service workers")
+    ap.add_argument("--batching", default="on", choices=("on", "off"),
+                    help="cfg4/cfg5: batched inner solves across requests")
     ap.add_argument("--hbm-budget", type=float, default=200e9)
     ap.add_argument("--tune-table", default=None,
                     help="launch-shape table to load (default: the one beside the library; "

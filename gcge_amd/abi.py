@@ -200,6 +200,9 @@ EXPORTS = {
     "gcge_service_wait": (C.c_int, [C.c_void_p, C.c_int64]),
     "gcge_service_stats": (None, [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int),
                                   C.POINTER(C.c_int64)]),
+    "gcge_service_set_batching": (None, [C.c_void_p, C.c_int]),
+    "gcge_service_batch_stats": (None, [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64),
+                                        c_int_p]),
     "gcge_service_destroy": (None, [C.c_void_p]),
     "gcge_eps_parameters": (C.c_int, [C.c_int, C.c_int, c_int_p, c_int_p, c_int_p]),
     "gcge_eps_solve": (C.c_int, [C.POINTER(CCSMat), C.POINTER(CCSMat), C.POINTER(EpsOptions),

@@ -34,6 +34,10 @@
 // receives the same steps in the same order as without the deferral.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <condition_variable>
+#include <mutex>
+#include <thread>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -234,8 +238,8 @@ __global__ __launch_bounds__(kWG) void k_zero_tiles(int64_t n, float *__restrict
 // At most 128 VGPRs (four waves per SIMD) except with B's entries gathered
 // separately, where the entry offsets and values of both matrices do not
 // fit: the unbounded 32-entry kernel took 130 and ran three waves.
-template <int W, bool TWO, bool XC>
-__global__ __launch_bounds__(kWG, (W == 32 && TWO) ? 1 : 4) void k_spmm_pw(
+template <int W, bool TWO>
+__device__ __forceinline__ void spmm_pw_body(int64_t rb, int64_t nblk, 
     int64_t n, const int64_t *__restrict__ slice_ptr, const int *__restrict__ colA,
     const float *__restrict__ valA, const int *__restrict__ colB,
     const float *__restrict__ valB, float sigma, const int *__restrict__ tiles,
@@ -245,10 +249,6 @@ __global__ __launch_bounds__(kWG, (W == 32 && TWO) ? 1 : 4) void k_spmm_pw(
     static_assert(kWG == 256, "one wavefront per slice, four per workgroup");
     __shared__ float red[(kWG / 64 + 1) * kC];
     const int ntact = *ntact_p;
-    const int64_t nblk = gridDim.x;
-    // partial sums are stored by row block, so their order (and the
-    // rounding of their sum) does not depend on XC
-    const int64_t rb = row_block<XC>();
     const int64_t row = rb * kWG + threadIdx.x;
     const bool live = row < n;
     // the wavefront's slice width; every lane of a slice has stored entries
@@ -319,20 +319,34 @@ __global__ __launch_bounds__(kWG, (W == 32 && TWO) ? 1 : 4) void k_spmm_pw(
     }
 }
 
+template <int W, bool TWO, bool XC>
+__global__ __launch_bounds__(kWG, (W == 32 && TWO) ? 1 : 4) void k_spmm_pw(
+    int64_t n, const int64_t *__restrict__ slice_ptr, const int *__restrict__ colA,
+    const float *__restrict__ valA, const int *__restrict__ colB,
+    const float *__restrict__ valB, float sigma, const int *__restrict__ tiles,
+    const int *__restrict__ ntact_p, const float *__restrict__ P, float *__restrict__ Wv,
+    float *__restrict__ partial)
+{
+    // partial sums are stored by row block, so their order (and the
+    // rounding of their sum) does not depend on XC
+    spmm_pw_body<W, TWO>(row_block<XC>(), gridDim.x, n, slice_ptr, colA, valA, colB, valB, sigma,
+                         tiles, ntact_p, P, Wv, partial);
+}
+
 // p = r + beta p on the listed tiles; p = r on the first iteration (p is
 // overwritten without being read: the workspace may hold anything).
 // defer: also d += alpha p with the p being replaced and the previous
 // iteration's alpha, on the listed tiles and on the `dropped` tiles the last
 // finish took off the list (those get only the d step).  only_d: the d step
 // alone on both sets (the step of the last iteration, after the loop).
-__global__ __launch_bounds__(kWG) void k_update_p(
+__device__ __forceinline__ void update_p_body(int64_t bx, int64_t gx, int by, 
     int64_t n, const int *__restrict__ tiles, const int *__restrict__ ntact_p,
     const int *__restrict__ dropped, const int *__restrict__ ndropped_p,
     const float *__restrict__ rho, const float *__restrict__ rho_old,
     const float *__restrict__ alpha, int first, int defer, int only_d,
     const float *__restrict__ R, float *__restrict__ P, float *__restrict__ D)
 {
-    const int k = blockIdx.y;
+    const int k = by;
     const int na = *ntact_p;
     int tile;
     bool step_p;
@@ -354,8 +368,8 @@ __global__ __launch_bounds__(kWG) void k_update_p(
         a[t] = step_d ? alpha[c] : 0.f;
     }
     const int64_t o = (int64_t)tile * n * kC;
-    for (int64_t i = (int64_t)blockIdx.x * kWG + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * kWG) {
+    for (int64_t i = (int64_t)bx * kWG + threadIdx.x; i < n;
+         i += (int64_t)gx * kWG) {
         if (first) {
             store8(P + o + i * kC, load8(R + o + i * kC));
             continue;
@@ -374,6 +388,17 @@ __global__ __launch_bounds__(kWG) void k_update_p(
             store8(P + o + i * kC, p);
         }
     }
+}
+
+__global__ __launch_bounds__(kWG) void k_update_p(
+    int64_t n, const int *__restrict__ tiles, const int *__restrict__ ntact_p,
+    const int *__restrict__ dropped, const int *__restrict__ ndropped_p,
+    const float *__restrict__ rho, const float *__restrict__ rho_old,
+    const float *__restrict__ alpha, int first, int defer, int only_d,
+    const float *__restrict__ R, float *__restrict__ P, float *__restrict__ D)
+{
+    update_p_body(blockIdx.x, gridDim.x, blockIdx.y, n, tiles, ntact_p, dropped, ndropped_p, rho,
+                  rho_old, alpha, first, defer, only_d, R, P, D);
 }
 
 // r -= alpha w on the listed tiles, with r.r partial sums (the d step of
@@ -562,7 +587,7 @@ __global__ __launch_bounds__(1024) void k_finish_rr(int nc, int mode,
 // ---------------------------------------------------------------------
 constexpr int64_t kFuseMaxRows = 65536;
 
-__global__ __launch_bounds__(kWG) void k_update_xr_pw(
+__device__ __forceinline__ void update_xr_pw_body(int64_t bx, int64_t gx, int by, 
     int64_t n, const int *__restrict__ tiles, const int *__restrict__ ntact_p, int nparts_pw,
     const float *__restrict__ partial_pw, const float *__restrict__ rho,
     const int *__restrict__ active, const float *__restrict__ P, const float *__restrict__ Wv,
@@ -570,7 +595,7 @@ __global__ __launch_bounds__(kWG) void k_update_xr_pw(
 {
     __shared__ float red[(kWG / 64 + 1) * kC];
     __shared__ float s_alpha[kC];
-    const int k = blockIdx.y;
+    const int k = by;
     if (k >= *ntact_p) return;
     const int tile = tiles[k];
     // alpha of the tile's columns, one wavefront per column
@@ -591,8 +616,8 @@ __global__ __launch_bounds__(kWG) void k_update_xr_pw(
         sq[t] = 0.f;
     }
     const int64_t o = (int64_t)tile * n * kC;
-    for (int64_t i = (int64_t)blockIdx.x * kWG + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * kWG) {
+    for (int64_t i = (int64_t)bx * kWG + threadIdx.x; i < n;
+         i += (int64_t)gx * kWG) {
         const f8 p = load8(P + o + i * kC), w = load8(Wv + o + i * kC);
         f8 d = load8(D + o + i * kC), r = load8(R + o + i * kC);
 #pragma unroll
@@ -606,8 +631,18 @@ __global__ __launch_bounds__(kWG) void k_update_xr_pw(
     }
     block_sum8(sq, red);
     if (threadIdx.x < kC)
-        partial_rr[(int64_t)(tile * kC + threadIdx.x) * gridDim.x + blockIdx.x] =
+        partial_rr[(int64_t)(tile * kC + threadIdx.x) * gx + bx] =
             red[kWG / 64 * kC + threadIdx.x];
+}
+
+__global__ __launch_bounds__(kWG) void k_update_xr_pw(
+    int64_t n, const int *__restrict__ tiles, const int *__restrict__ ntact_p, int nparts_pw,
+    const float *__restrict__ partial_pw, const float *__restrict__ rho,
+    const int *__restrict__ active, const float *__restrict__ P, const float *__restrict__ Wv,
+    float *__restrict__ D, float *__restrict__ R, float *__restrict__ partial_rr)
+{
+    update_xr_pw_body(blockIdx.x, gridDim.x, blockIdx.y, n, tiles, ntact_p, nparts_pw, partial_pw,
+                      rho, active, P, Wv, D, R, partial_rr);
 }
 
 // rho and stop flag of column c from k_update_xr_pw's partials, as
@@ -630,7 +665,7 @@ __device__ __forceinline__ void next_state(int c, int nparts_rr,
     }
 }
 
-__global__ __launch_bounds__(kWG) void k_update_p_rr(
+__device__ __forceinline__ void update_p_rr_body(int64_t bx, int64_t gx, int by, 
     int64_t n, int nc, int nparts_rr, const float *__restrict__ partial_rr,
     const int *__restrict__ tiles_o, const int *__restrict__ ntact_o,
     const float *__restrict__ rho_o, const int *__restrict__ active_o,
@@ -642,7 +677,7 @@ __global__ __launch_bounds__(kWG) void k_update_p_rr(
     __shared__ float s_beta[kC];
     __shared__ int s_on;
     const int ntile = (nc + kC - 1) / kC;
-    if (blockIdx.x == 0 && blockIdx.y == 0) {
+    if (bx == 0 && by == 0) {
         for (int t = threadIdx.x; t < ntile; t += kWG) tile_on[t] = 0;
         __syncthreads();
         for (int c = threadIdx.x; c < ntile * kC; c += kWG) {
@@ -661,7 +696,7 @@ __global__ __launch_bounds__(kWG) void k_update_p_rr(
             *ntact_n = q;
         }
     }
-    const int k = blockIdx.y;
+    const int k = by;
     if (k >= *ntact_o) return;
     const int tile = tiles_o[k];
     if (threadIdx.x == 0) s_on = 0;
@@ -680,13 +715,110 @@ __global__ __launch_bounds__(kWG) void k_update_p_rr(
 #pragma unroll
     for (int t = 0; t < kC; ++t) beta[t] = s_beta[t];
     const int64_t o = (int64_t)tile * n * kC;
-    for (int64_t i = (int64_t)blockIdx.x * kWG + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * kWG) {
+    for (int64_t i = (int64_t)bx * kWG + threadIdx.x; i < n;
+         i += (int64_t)gx * kWG) {
         const f8 r = load8(R + o + i * kC);
         f8 p = load8(P + o + i * kC);
 #pragma unroll
         for (int t = 0; t < kC; ++t) p.v[t] = r.v[t] + beta[t] * p.v[t];
         store8(P + o + i * kC, p);
+    }
+}
+
+__global__ __launch_bounds__(kWG) void k_update_p_rr(
+    int64_t n, int nc, int nparts_rr, const float *__restrict__ partial_rr,
+    const int *__restrict__ tiles_o, const int *__restrict__ ntact_o,
+    const float *__restrict__ rho_o, const int *__restrict__ active_o,
+    int *__restrict__ tiles_n, int *__restrict__ ntact_n, float *__restrict__ rho_n,
+    int *__restrict__ active_n, const float *__restrict__ init_res, float rate, float tol,
+    const float *__restrict__ R, float *__restrict__ P)
+{
+    update_p_rr_body(blockIdx.x, gridDim.x, blockIdx.y, n, nc, nparts_rr, partial_rr, tiles_o,
+                     ntact_o, rho_o, active_o, tiles_n, ntact_n, rho_n, active_n, init_res, rate,
+                     tol, R, P);
+}
+
+// ---------------------------------------------------------------------
+// Batched BlockPCG iterations across solves.  A batcher thread (PcgBatcher
+// below) runs the three-launch iteration for every solve registered with
+// it: one launch per step serves all of them, the solve of workgroup
+// (x, y, z) being job z of a table uploaded each round.  Each job runs the
+// bodies above with its own launch extents (nb, nblk_row) in place of the
+// grid's, its own buffers and its own iteration parity, so its arithmetic
+// is that of the same solve run alone, bit for bit.  Grid extents are the
+// largest over the jobs; workgroups past their job's extents leave at once.
+// ---------------------------------------------------------------------
+struct BatchJob {
+    int64_t n;
+    const int64_t *slice_ptr;
+    const int *col;
+    const float *val;
+    float *P, *Wv, *D, *R, *partial, *partial_rr;
+    int *tiles[2], *ntact[2], *active[2];
+    float *rho[2];
+    const float *init_res;
+    float rate, tol;
+    int nc, nb, nblk_row, nt;  // nt: bound on the tiles listed in this group of rounds
+    int it0, max_iter;         // iterations run before the group; the limit
+};
+
+// p = r (first iteration) or p = r + beta p with the state of the previous
+// iteration folded in (k_update_p / k_update_p_rr of the fused path)
+// Round r of a group runs iteration it0 + r of each job (none past its
+// max_iter); its parity picks the state buffers.
+__global__ __launch_bounds__(kWG) void k_batch_update_p(const BatchJob *__restrict__ jobs, int r)
+{
+    const BatchJob &j = jobs[blockIdx.z];
+    const int it = j.it0 + r;
+    if ((int)blockIdx.x >= j.nb || (int)blockIdx.y >= j.nt || it >= j.max_iter)
+        return;  // workgroup-uniform
+    const int par = it & 1;
+    if (it == 0) {
+        update_p_body(blockIdx.x, j.nb, blockIdx.y, j.n, j.tiles[0], j.ntact[0], nullptr, nullptr,
+                      j.rho[0], j.rho[0], nullptr, 1, 0, 0, j.R, j.P, j.D);
+    } else {
+        const int o = par ^ 1, q = par;
+        update_p_rr_body(blockIdx.x, j.nb, blockIdx.y, j.n, j.nc, j.nb, j.partial_rr, j.tiles[o],
+                         j.ntact[o], j.rho[o], j.active[o], j.tiles[q], j.ntact[q], j.rho[q],
+                         j.active[q], j.init_res, j.rate, j.tol, j.R, j.P);
+    }
+}
+
+// w = A p with the partial p.w sums (k_spmm_pw, rows of at most 8 entries,
+// no shift)
+__global__ __launch_bounds__(kWG, 4) void k_batch_spmm_pw(const BatchJob *__restrict__ jobs, int r)
+{
+    const BatchJob &j = jobs[blockIdx.z];
+    const int it = j.it0 + r;
+    if ((int)blockIdx.x >= j.nblk_row || it >= j.max_iter) return;  // workgroup-uniform
+    const int par = it & 1;
+    spmm_pw_body<8, false>(blockIdx.x, j.nblk_row, j.n, j.slice_ptr, j.col, j.val, j.col, nullptr,
+                           0.f, j.tiles[par], j.ntact[par], j.P, j.Wv, j.partial);
+}
+
+// alpha, d += alpha p, r -= alpha w with the r.r partials (k_update_xr_pw)
+__global__ __launch_bounds__(kWG) void k_batch_update_xr(const BatchJob *__restrict__ jobs, int r)
+{
+    const BatchJob &j = jobs[blockIdx.z];
+    const int it = j.it0 + r;
+    if ((int)blockIdx.x >= j.nb || (int)blockIdx.y >= j.nt || it >= j.max_iter)
+        return;  // workgroup-uniform
+    const int q = it & 1;
+    update_xr_pw_body(blockIdx.x, j.nb, blockIdx.y, j.n, j.tiles[q], j.ntact[q], j.nblk_row,
+                      j.partial, j.rho[q], j.active[q], j.P, j.Wv, j.D, j.R, j.partial_rr);
+}
+
+// counts[z] = the active-tile count of job z's latest state after a group
+// of `rounds` rounds (the state of the iteration before its last one, as
+// the solo loop reads it)
+__global__ void k_batch_counts(const BatchJob *__restrict__ jobs, int njobs, int rounds,
+                               int *__restrict__ counts)
+{
+    const int z = (int)threadIdx.x;
+    if (z < njobs) {
+        const BatchJob &j = jobs[z];
+        const int last = min(j.it0 + rounds, j.max_iter) - 1;
+        counts[z] = *j.ntact[last & 1];
     }
 }
 
@@ -710,6 +842,8 @@ struct PCG32 {
     int *h_ntact = nullptr;
     int64_t iters = 0;
     bool use_fused = true;  // three launches per iteration up to kFuseMaxRows rows
+    // the batcher this solver's iterations join (service workers), or null
+    struct PcgBatcher *batcher = nullptr;
     // kCheck inner iterations captured as one graph, for one key
     bool use_graphs = true;
     hipGraphExec_t gexec = nullptr;
@@ -794,6 +928,140 @@ unsigned blocks_for(int64_t n, int cap)
     return (unsigned)(b < 1 ? 1 : b);
 }
 
+// The batcher: a thread that runs rounds of batched iterations over the
+// solves submitted to it, kCheck rounds per group.  A solve is submitted
+// after its initial residual and state are in place (the submitting thread
+// has waited for its stream); it joins the next group, leaves once its
+// active-tile count reads 0 or it has run max_iter iterations, gets
+// x += d, and its caller is woken.  The job table is uploaded once per
+// group and the counts of all the jobs are read after it (one copy each
+// way and one wait per group); between reads the kernels skip the tiles
+// that have stopped, so rounds after a job's last active iteration change
+// nothing.
+struct PcgBatcher {
+    struct Job {
+        BatchJob d;
+        int it = 0, max_iter = 0, nt = 0;
+        double *x64 = nullptr;
+        int64_t ldx = 0;
+        int64_t *iters = nullptr;
+        bool done = false;
+    };
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    std::vector<Job *> incoming;
+    bool stop = false;
+    std::thread th;
+    int64_t rounds = 0, job_iters = 0;
+    int max_jobs_seen = 0;
+
+    PcgBatcher() { th = std::thread([this] { run(); }); }
+    ~PcgBatcher()
+    {
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            stop = true;
+        }
+        cv_work.notify_all();
+        th.join();
+    }
+
+    // runs the job's iterations with the others'; returns when done
+    void solve(Job &j)
+    {
+        std::unique_lock<std::mutex> lk(mu);
+        incoming.push_back(&j);
+        cv_work.notify_all();
+        cv_done.wait(lk, [&] { return j.done; });
+    }
+
+    void run()
+    {
+        constexpr int kMaxJobs = 256;
+        Ctx &c = ctx();
+        hipStream_t st = c.stream;
+        BatchJob *h_tab = nullptr, *d_tab = nullptr;
+        int *h_cnt = nullptr, *d_cnt = nullptr;
+        pinned_alloc(&h_tab, sizeof(BatchJob) * kMaxJobs);
+        dev_alloc(&d_tab, sizeof(BatchJob) * kMaxJobs);
+        pinned_alloc(&h_cnt, sizeof(int) * kMaxJobs);
+        dev_alloc(&d_cnt, sizeof(int) * kMaxJobs);
+        std::vector<Job *> active;
+        for (;;) {
+            {
+                std::unique_lock<std::mutex> lk(mu);
+                cv_work.wait(lk, [&] { return stop || !incoming.empty() || !active.empty(); });
+                if (stop && incoming.empty() && active.empty()) break;
+                while (!incoming.empty() && (int)active.size() < kMaxJobs) {
+                    active.push_back(incoming.back());
+                    incoming.pop_back();
+                }
+            }
+            const int nj = (int)active.size();
+            if (nj > max_jobs_seen) max_jobs_seen = nj;
+            int gx = 1, gy = 1, gr = 1, nr = 1;
+            for (int z = 0; z < nj; ++z) {
+                Job &j = *active[z];
+                j.d.it0 = j.it;
+                j.d.max_iter = j.max_iter;
+                j.d.nt = j.nt;
+                h_tab[z] = j.d;
+                gx = std::max(gx, j.d.nb);
+                gy = std::max(gy, j.nt);
+                gr = std::max(gr, j.d.nblk_row);
+                nr = std::max(nr, std::min(kCheck, j.max_iter - j.it));
+            }
+            HIPCHECK(hipMemcpyAsync(d_tab, h_tab, sizeof(BatchJob) * nj, hipMemcpyHostToDevice, st));
+            for (int r = 0; r < nr; ++r) {
+                k_batch_update_p<<<dim3(gx, gy, nj), kWG, 0, st>>>(d_tab, r);
+                k_batch_spmm_pw<<<dim3(gr, 1, nj), kWG, 0, st>>>(d_tab, r);
+                k_batch_update_xr<<<dim3(gx, gy, nj), kWG, 0, st>>>(d_tab, r);
+            }
+            k_batch_counts<<<1, kMaxJobs, 0, st>>>(d_tab, nj, nr, d_cnt);
+            HIPCHECK(hipMemcpyAsync(h_cnt, d_cnt, sizeof(int) * nj, hipMemcpyDeviceToHost, st));
+            c.sync();
+            rounds += nr;
+            std::vector<Job *> finished;
+            for (int z = 0; z < nj; ++z) {
+                Job &j = *active[z];
+                const int ran = std::min(nr, j.max_iter - j.it);
+                job_iters += ran;
+                j.it += ran;
+                j.nt = h_cnt[z];
+                if (j.nt == 0 || j.it >= j.max_iter) finished.push_back(&j);
+            }
+            if (finished.empty()) continue;
+            for (Job *j : finished) {
+                const unsigned nblk = (unsigned)((j->d.n + kWG - 1) / kWG);
+                const int ntile = (j->d.nc + kC - 1) / kC;
+                k_add_up<<<dim3(nblk, ntile), kWG, 0, st>>>(j->d.n, j->d.nc, j->d.D, j->x64,
+                                                            j->ldx);
+                *j->iters += j->it;
+            }
+            c.sync();
+            // a Job lives in its caller's frame, which is gone once the
+            // caller wakes: leave the finished ones out of the list before
+            // they are released and never touch them afterwards
+            active.erase(std::remove_if(active.begin(), active.end(),
+                                        [&](Job *j) {
+                                            return std::find(finished.begin(), finished.end(),
+                                                             j) != finished.end();
+                                        }),
+                         active.end());
+            {
+                std::lock_guard<std::mutex> lk(mu);
+                for (Job *j : finished) j->done = true;
+            }
+            cv_done.notify_all();
+        }
+        pinned_free(h_tab);
+        dev_free(d_tab);
+        pinned_free(h_cnt);
+        dev_free(d_cnt);
+        release_ctx();
+    }
+};
+
 void pcg32_solve(void *matA, void **mv_b, void **mv_x, int *start, int *end, OPS *ops)
 {
     PCG32 &S = *static_cast<PCG32 *>(ops->multi_linear_solver_workspace);
@@ -876,6 +1144,48 @@ void pcg32_solve(void *matA, void **mv_b, void **mv_x, int *start, int *end, OPS
     // (stencil.hip), with its own count of p.w partial sums per column
     const int sparts = sigma == 0.f ? stencil_pw_parts(A->sell) : 0;
     const int nparts_pw = sparts > 0 ? sparts : (int)nblk_row;
+    // served solves of the fused kind (general rows of at most 8 entries,
+    // no shift) run their iterations in the batcher's rounds
+    if (S.batcher != nullptr && S.use_fused && n <= kFuseMaxRows && sparts == 0 && !two &&
+        sigma == 0.f && A->sell.max_width <= 8 && !c.prof.on) {
+        PcgBatcher::Job j;
+        BatchJob &d = j.d;
+        d.n = n;
+        d.slice_ptr = A->sell.slice_ptr;
+        d.col = A->sell.col;
+        d.val = A->sell.val32;
+        d.P = S.p;
+        d.Wv = S.w;
+        d.D = S.x;
+        d.R = S.r;
+        d.partial = S.partial;
+        d.partial_rr = S.partial_rr;
+        d.tiles[0] = S.tiles;
+        d.tiles[1] = S.tiles_b;
+        d.ntact[0] = S.ntact;
+        d.ntact[1] = S.ntact_b;
+        d.active[0] = S.active;
+        d.active[1] = S.active_b;
+        d.rho[0] = S.rho;
+        d.rho[1] = S.rho_b;
+        d.init_res = S.init_res;
+        d.rate = (float)S.rate;
+        d.tol = (float)S.tol;
+        d.nc = nc;
+        d.nb = (int)nb;
+        d.nblk_row = (int)nblk_row;
+        j.max_iter = S.max_iter;
+        j.nt = ntact;
+        j.x64 = Xv->d + (int64_t)start[1] * Xv->ldd;
+        j.ldx = Xv->ldd;
+        j.iters = &S.iters;
+        if (ntact > 0 && S.max_iter > 0) {
+            S.batcher->solve(j);
+        } else {
+            k_add_up<<<dim3(nblk_row, ntile), kWG, 0, st>>>(n, nc, S.x, j.x64, j.ldx);
+        }
+        return;
+    }
     const double mat_bytes =
         sparts > 0 ? 0.0
                    : (double)A->sell.nnz_true * 8.0 + 4.0 * (double)(n + 1) +
@@ -1063,6 +1373,27 @@ void pcg32_solve(void *matA, void **mv_b, void **mv_x, int *start, int *end, OPS
 }
 
 }  // namespace
+
+void *pcg32_batcher_create() { return new PcgBatcher(); }
+
+void pcg32_batcher_destroy(void *b) { delete static_cast<PcgBatcher *>(b); }
+
+void pcg32_batcher_stats(void *b, int64_t *rounds, int64_t *job_iters, int *max_jobs)
+{
+    PcgBatcher *p = static_cast<PcgBatcher *>(b);
+    std::lock_guard<std::mutex> lk(p->mu);
+    if (rounds) *rounds = p->rounds;
+    if (job_iters) *job_iters = p->job_iters;
+    if (max_jobs) *max_jobs = p->max_jobs_seen;
+}
+
+void pcg32_set_batcher(OPS *ops, void *b)
+{
+    if (ops->MultiLinearSolver != pcg32_solve || ops->multi_linear_solver_workspace == nullptr)
+        fail("pcg32_set_batcher: no device BlockPCG installed", __FILE__, __LINE__);
+    static_cast<PCG32 *>(ops->multi_linear_solver_workspace)->batcher =
+        static_cast<PcgBatcher *>(b);
+}
 
 int64_t pcg32_iterations(OPS *ops)
 {

@@ -30,6 +30,16 @@
 
 #include "gcge_hip.h"
 
+namespace gcge {
+// The batched BlockPCG of pcg32.hip: one thread running the inner
+// iterations of every mixed-precision solve attached to it in shared
+// launches.
+void *pcg32_batcher_create();
+void pcg32_batcher_destroy(void *b);
+void pcg32_batcher_stats(void *b, int64_t *rounds, int64_t *job_iters, int *max_jobs);
+void pcg32_set_batcher(OPS *ops, void *b);
+}  // namespace gcge
+
 namespace {
 
 double now_s()
@@ -56,13 +66,17 @@ struct Service {
     int64_t completed = 0;
     bool stop = false;
     std::vector<std::thread> workers;
+    // the inner iterations of mixed requests run in one batcher's shared
+    // launches while batching is on (the default)
+    void *batcher = nullptr;
+    bool batching = true;
 };
 
 int64_t nnz_of(const gcge_ccs_mat *m) { return m ? (int64_t)m->j_col[m->ncols] : 0; }
 
 // One request on the calling thread's stream.  Returns 0 or a negative
 // status (matrix rejected).
-int run_one(Job &j)
+int run_one(Service *s, Job &j)
 {
     const gcge_request &q = j.req;
     gcge_result &r = *j.res;
@@ -94,6 +108,12 @@ int run_one(Job &j)
     HIPGCG_Setparameters(q.gap_min, mixed ? 0 : q.shift, ops);
     HIPGCG_SetOrthMethod(ops, mixed ? GCGE_ORTH_CHOLQR : GCGE_ORTH_MGS);
     if (mixed) HIPGCG_MultiLinearSolverSetup_BlockPCG32(30, 1e-2, 1e-14, B, 0.0, ops);
+    bool batched;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        batched = s->batching;
+    }
+    if (mixed && batched) gcge::pcg32_set_batcher(ops, s->batcher);
     std::vector<double> eval((size_t)q.nev_max, 0.0);
     int nconv = q.nev;
     // warm start: the given vectors in the first columns of evec, where
@@ -150,7 +170,7 @@ void worker(Service *s)
             j->res->queue_seconds = now_s() - j->t_submit;
             j->res->concurrent = s->running;
         }
-        const int st = run_one(*j);
+        const int st = run_one(s, *j);
         {
             std::lock_guard<std::mutex> lk(s->mu);
             j->res->status = st;
@@ -197,8 +217,23 @@ void *gcge_service_create(int workers, int64_t hbm_budget_bytes)
     if (workers < 1 || hbm_budget_bytes <= 0) return nullptr;
     Service *s = new Service();
     s->budget = hbm_budget_bytes;
+    s->batcher = gcge::pcg32_batcher_create();
     for (int i = 0; i < workers; ++i) s->workers.emplace_back(worker, s);
     return s;
+}
+
+void gcge_service_set_batching(void *svc, int on)
+{
+    Service *s = static_cast<Service *>(svc);
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->batching = on != 0;
+}
+
+void gcge_service_batch_stats(void *svc, int64_t *rounds, int64_t *solve_iterations,
+                              int *max_solves)
+{
+    Service *s = static_cast<Service *>(svc);
+    gcge::pcg32_batcher_stats(s->batcher, rounds, solve_iterations, max_solves);
 }
 
 int64_t gcge_service_submit(void *svc, const gcge_request *req, gcge_result *res)
@@ -262,6 +297,7 @@ void gcge_service_destroy(void *svc)
     }
     s->cv_work.notify_all();
     for (auto &t : s->workers) t.join();
+    gcge::pcg32_batcher_destroy(s->batcher);  // after the workers: no solve is waiting
     for (auto &kv : s->jobs) delete kv.second;
     delete s;
 }

@@ -41,11 +41,12 @@ def _ccs(M):
 
 
 class Service:
-    def __init__(self, workers: int = 8, hbm_budget: float = 200e9):
+    def __init__(self, workers: int = 8, hbm_budget: float = 200e9, batching: bool = True):
         self.lib = abi.load()
         self.h = self.lib.gcge_service_create(int(workers), int(hbm_budget))
         if not self.h:
             raise ValueError("gcge_service_create rejected its arguments")
+        self.lib.gcge_service_set_batching(self.h, 1 if batching else 0)
         self._live = {}
 
     @staticmethod
@@ -104,6 +105,13 @@ class Service:
         return ServedResult(ev[:k].copy(), k, res.num_iter, res.concurrent, res.queue_seconds,
                             res.setup_seconds, res.solve_seconds, res.total_seconds,
                             vec[:, :k].copy() if vec is not None else None, res.pool_bytes)
+
+    def batch_stats(self):
+        """Batched inner solves: rounds, request iterations they carried and
+        the most requests in one round."""
+        r, it, mx = C.c_int64(), C.c_int64(), C.c_int()
+        self.lib.gcge_service_batch_stats(self.h, C.byref(r), C.byref(it), C.byref(mx))
+        return {"rounds": r.value, "solve_iterations": it.value, "max_solves": mx.value}
 
     def stats(self):
         done, conc, used = C.c_int64(), C.c_int(), C.c_int64()

@@ -387,6 +387,16 @@ void gcge_service_stats(void *svc, int64_t *completed, int *max_concurrent,
                         int64_t *hbm_in_use);
 /* Finishes the queued requests and joins the workers. */
 void gcge_service_destroy(void *svc);
+/* Batched inner solves (on by default): the FP32 BlockPCG iterations of
+ * the mixed-precision requests in progress (systems of at most 65,536 rows
+ * with at most 8 entries per row) run in shared launches, one per step for
+ * all of them, driven by one service thread; each request gets the
+ * results it would get alone, bit for bit.  Applies to requests started
+ * after the call.  Stats: batched rounds, request iterations they carried,
+ * and the most requests one round carried. */
+void gcge_service_set_batching(void *svc, int on);
+void gcge_service_batch_stats(void *svc, int64_t *rounds, int64_t *solve_iterations,
+                              int *max_solves);
 
 /* ------------------------------------------------------------------ */
 /* EPS-style entry point                                               */

@@ -119,3 +119,29 @@ def test_worker_pool_stops_growing(hip, mode):
     for ps in passes[1:]:
         for a, b in zip(passes[0], ps):
             assert np.array_equal(a.eval, b.eval) and a.num_iter == b.num_iter
+
+
+def test_batched_inner_solves_equal_solo(hip):
+    """With batching on, the BlockPCG iterations of concurrent mixed requests
+    run in shared launches (one per step for all of them); every request
+    gets the same eigenvalues and iteration count as with batching off, bit
+    for bit, and rounds do carry several requests.  A 3D request too large
+    for the batched kernels (over 65,536 rows) runs its own loop in the
+    same mix."""
+    probs = [P.config4_problem(i) for i in range(8)] + [P.stencil_problem(42, 3, 10)]
+    runs = {}
+    for batching in (False, True):
+        with Service(workers=6, batching=batching) as svc:
+            tickets = [svc.submit(p, mode="mixed", seed=40 + i) for i, p in enumerate(probs)]
+            runs[batching] = [svc.wait(t) for t in tickets]
+            bst = svc.batch_stats()
+        if batching:
+            print(f"# batched rounds {bst}")
+            assert bst["rounds"] > 0 and bst["max_solves"] >= 2
+            assert bst["solve_iterations"] > bst["rounds"]
+        else:
+            assert bst["rounds"] == 0
+    for p, a, b in zip(probs, runs[False], runs[True]):
+        assert a.nev_conv == b.nev_conv and a.num_iter == b.num_iter, p.name
+        assert np.array_equal(a.eval, b.eval), p.name
+        assert a.nev_conv >= p.nev
