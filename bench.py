@@ -93,8 +93,12 @@ def make_grid_problem(name: str):
 def run_path(prob, n, dim, mode, steps, warmup, mats, want_check=True):
     from gcge_amd import abi, problems as P, solver as S
     lib = abi.load()
-    shift = 1 if mode == "fp64" else 0
-    kw = dict(shift=shift, mode=mode, rand="device", seed=1, mats=mats, print_level=1)
+    # fp64: the reference's algorithm (autoshift, column-by-column MGS);
+    # fp64_cholqr: the same with the block Cholesky QR orthogonalization;
+    # mixed: FP32 BlockPCG without shift, Cholesky QR
+    shift = 0 if mode == "mixed" else 1
+    kw = dict(shift=shift, mode="mixed" if mode == "mixed" else "fp64", rand="device", seed=1,
+              mats=mats, print_level=1, orth="cholqr" if mode != "fp64" else "mgs")
     for w in range(warmup):
         log(f"{mode}: warmup {w + 1}/{warmup}")
         S.solve_problem(prob, **kw)
@@ -108,17 +112,7 @@ def run_path(prob, n, dim, mode, steps, warmup, mats, want_check=True):
         log(f"{mode}: step {k + 1}/{steps}: {res.seconds:.3f} s, iters {res.num_iter}, "
             f"nconv {res.nev_conv}, bpcg {res.bpcg_iters}")
     lib.gcge_hip_profile_enable(0)
-    prof = {}
-    import ctypes as C
-    for kind, name in enumerate(("spmm64", "spmm32", "gram", "lincomb", "blas1", "pcgvec")):
-        la, ms, by, fl = C.c_int64(), C.c_double(), C.c_double(), C.c_double()
-        lib.gcge_hip_profile_read(kind, C.byref(la), C.byref(ms), C.byref(by), C.byref(fl))
-        if la.value:
-            prof[name] = {"launches": la.value, "total_ms": ms.value,
-                          "avg_us": 1e3 * ms.value / la.value,
-                          "bytes": by.value, "flops": fl.value,
-                          "GBs": by.value / (ms.value * 1e-3) / 1e9 if ms.value else 0.0,
-                          "TFs": fl.value / (ms.value * 1e-3) / 1e12 if ms.value else 0.0}
+    prof = prof_totals(lib.gcge_hip_profile_read)
     k = res.nev_conv
     out = {"seconds": secs, "num_iter": res.num_iter, "nev_conv": k,
            "bpcg_iters": res.bpcg_iters, "phases": res.phases, "kernels": prof}
@@ -322,35 +316,109 @@ def measured_traffic(kernel: str, config: str):
     return None
 
 
-def cpu_baseline(prob, gpu_iters, nev_conv, iters_sample):
-    """Reference GCGE (OpenMP build) on the same workload for a bounded
-    number of outer iterations, in a child process; extrapolated."""
+def _ref_sample(what: str, size: int):
+    """Runs oracle/ref_sample.py (the compiled reference on the CPU) in a
+    child process; returns (json, cores) or (None, reason)."""
     refso = os.path.join(ROOT, "oracle", "_ref", "libgcge_ref_omp.so")
     if not os.path.exists(refso):
         return None, "oracle/_ref/libgcge_ref_omp.so not built"
     cores = min(os.cpu_count() or 1, 16)
     env = dict(os.environ, OMP_NUM_THREADS=str(cores), MKL_NUM_THREADS="1")
-    cmd = [sys.executable, os.path.join(ROOT, "oracle", "ref_sample.py"), refso,
-           str(iters_sample)]
-    log(f"cpu baseline: reference GCGE, {cores} threads, {iters_sample} outer iteration(s)")
-    try:
-        p = subprocess.run(cmd, env=env, capture_output=True, text=True, timeout=900)
-    except subprocess.TimeoutExpired:
-        return None, "reference sample timed out"
-    if p.returncode != 0:
-        return None, f"reference sample failed: {p.stderr[-300:]}"
-    r = json.loads(p.stdout.strip().splitlines()[-1])
+    cmd = [sys.executable, os.path.join(ROOT, "oracle", "ref_sample.py"), refso, what,
+           str(size)]
+    log(f"cpu baseline: reference GCGE, {cores} threads, {what} sample of size {size}")
+    # the reference's own monitor lines come through as they are printed
+    p = subprocess.Popen(cmd, env=env, stdout=subprocess.PIPE, stderr=subprocess.STDOUT,
+                         text=True)
+    last = None
+    t0 = time.perf_counter()
+    for ln in p.stdout:
+        ln = ln.rstrip()
+        if ln.startswith("{"):
+            last = ln
+        elif ln:
+            log(f"reference: {ln[:160]}")
+        if time.perf_counter() - t0 > 1200:
+            p.kill()
+            return None, "reference sample timed out"
+    if p.wait() != 0 or last is None:
+        return None, "reference sample failed"
+    r = json.loads(last)
+    r["cores"] = cores
+    return r, None
+
+
+def cpu_baseline(config, gpu_iters, nev_conv, iters_sample):
+    """Reference GCGE (OpenMP build) on the same single-problem workload for
+    a bounded number of outer iterations; extrapolated to the GPU run's
+    iteration count."""
+    r, why = _ref_sample(config, iters_sample)
+    if r is None:
+        return None, why
     t_init = r["phases"]["initX"]
     t_sample = r["seconds"]
     per_iter = (t_sample - t_init) / max(r["num_iter"], 1)
     t_full = t_init + per_iter * gpu_iters
     val = nev_conv / t_full
-    return {"value": val, "unit": "eigenpairs/s", "cores": cores, "kind": "reference",
+    size = "128^3" if config == "cfg2" else "160^3 27-point"
+    return {"value": val, "unit": "eigenpairs/s", "cores": r["cores"], "kind": "reference",
             "sample": (f"reference GCGE (OpenMP build, MKL BLAS/LAPACK 1 thread) on the same "
-                       f"128^3 problem: InitializeX + {r['num_iter']} outer iteration(s) in "
+                       f"{size} problem: InitializeX + {r['num_iter']} outer iteration(s) in "
                        f"{t_sample:.1f} s (initX {t_init:.1f} s, {per_iter:.1f} s/iteration), "
                        f"extrapolated to the GPU run's {gpu_iters} iterations"),
             "seconds_extrapolated": t_full}, None
+
+
+def cpu_baseline_served(config, count):
+    """Reference GCGE on the first `count` problems of the batched/served
+    workload, solved one after another on all the host cores."""
+    r, why = _ref_sample(config, count)
+    if r is None:
+        return None, why
+    return {"value": r["nev_conv"] / r["seconds"], "unit": "eigenpairs/s", "cores": r["cores"],
+            "kind": "reference",
+            "sample": (f"reference GCGE (OpenMP build, FP64, autoshift on) on the first "
+                       f"{r['problems']} problems of the same workload, one after another: "
+                       f"{r['nev_conv']} pairs in {r['seconds']:.1f} s "
+                       f"({r['num_iter']} outer iterations)")}, None
+
+
+PROF_KINDS = ("spmm64", "spmm32", "gram", "lincomb", "blas1", "pcgvec")
+
+
+def roofline_of(kern, config):
+    """The roofline object of the dominant kernel class (by total time)."""
+    dom = max(kern, key=lambda k: kern[k]["total_ms"]) if kern else None
+    if not dom:
+        return None
+    kd = kern[dom]
+    # FP64 ridge point: 78.6 TFLOP/s / 8 TB/s = 9.8 flop/byte
+    intensity = kd["flops"] / max(kd["bytes"], 1.0)
+    common = {"kernel": dom, "avg_launch_us": kd["avg_us"], "launches": kd["launches"],
+              "algorithmic_bytes_per_launch": kd["bytes"] / kd["launches"],
+              "algorithmic_flops_per_launch": kd["flops"] / kd["launches"],
+              "flop_per_byte": intensity,
+              "traffic": measured_traffic(dom, config)}
+    if intensity > FP64_PEAK_TFS / (HBM_PEAK_GBS / 1e3):
+        return {"bound": "mfma", "achieved": kd["TFs"], "peak": FP64_PEAK_TFS,
+                "unit": "TFLOP/s", "frac": kd["TFs"] / FP64_PEAK_TFS, **common}
+    return {"bound": "hbm", "achieved": kd["GBs"], "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": kd["GBs"] / HBM_PEAK_GBS, **common}
+
+
+def prof_totals(read):
+    import ctypes as C
+    prof = {}
+    for kind, name in enumerate(PROF_KINDS):
+        la, ms, by, fl = C.c_int64(), C.c_double(), C.c_double(), C.c_double()
+        read(kind, C.byref(la), C.byref(ms), C.byref(by), C.byref(fl))
+        if la.value:
+            prof[name] = {"launches": la.value, "total_ms": ms.value,
+                          "avg_us": 1e3 * ms.value / la.value,
+                          "bytes": by.value, "flops": fl.value,
+                          "GBs": by.value / (ms.value * 1e-3) / 1e9 if ms.value else 0.0,
+                          "TFs": fl.value / (ms.value * 1e-3) / 1e12 if ms.value else 0.0}
+    return prof
 
 
 def served(args):
@@ -371,6 +439,9 @@ def served(args):
     log(f"{args.config}: generated {len(items)} problems in {time.perf_counter() - t0:.1f}s")
     check_every = max(1, len(items) // 8)
     res = [None] * len(items)
+    from gcge_amd import abi
+    lib = abi.load()
+    lib.gcge_hip_profile_all(1)  # every worker's and the batcher's launches, timed
     with Service(workers=args.workers, hbm_budget=args.hbm_budget,
                  batching=args.batching == "on") as svc:
         # warm the workers' kernels and handles on a small problem each
@@ -390,6 +461,12 @@ def served(args):
         wall = time.perf_counter() - start
         st = svc.stats()
         bst = svc.batch_stats()
+    kern = prof_totals(lib.gcge_hip_profile_read_all)  # the workers have been released
+    lib.gcge_hip_profile_all(0)
+    roof = roofline_of(kern, args.config)
+    cpu, why = (None, "disabled")
+    if args.cpu_baseline == "on":
+        cpu, why = cpu_baseline_served(args.config, args.cpu_count)
     lat = np.array([r.total_seconds for r in res])
     conv = sum(r.nev_conv for r in res)
     short = sum(1 for (_, p), r in zip(items, res) if r.nev_conv < p.nev)
@@ -417,10 +494,12 @@ def served(args):
         "max_rel_residual_sampled": worst,
         "queue_s_p50": float(np.percentile([r.queue_seconds for r in res], 50)),
         "solve_s_p50": float(np.percentile([r.solve_seconds for r in res], 50)),
-        "roofline": None, "cpu_baseline": None,
-        "cpu_baseline_note": "not measured for the batched/served configurations",
+        "kernels": kern,
+        "roofline": roof, "cpu_baseline": cpu,
         "tuning_entries": tune_entries(),
     }
+    if cpu is None:
+        line["cpu_baseline_note"] = why
     print(json.dumps(line))
 
 
@@ -430,7 +509,9 @@ def main():
     ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", default="cfg2")
-    ap.add_argument("--modes", default="fp64,mixed")
+    ap.add_argument("--modes", default="fp64,fp64_cholqr,mixed")
+    ap.add_argument("--cpu-count", type=int, default=6,
+                    help="cfg4/cfg5: problems in the reference's CPU sample")
     ap.add_argument("--cpu-baseline", default="on", choices=("on", "off"))
     ap.add_argument("--cpu-iters", type=int, default=1)
     ap.add_argument("--copy-probe", action="store_true")
@@ -513,27 +594,10 @@ def main():
     secs = np.array(hr["seconds"])
     value = hr["nev_conv"] / secs.mean()
     # dominant kernel class of the headline path
-    kern = hr["kernels"]
-    dom = max(kern, key=lambda k: kern[k]["total_ms"]) if kern else None
-    roof = None
-    if dom:
-        kd = kern[dom]
-        # FP64 ridge point: 78.6 TFLOP/s / 8 TB/s = 9.8 flop/byte
-        intensity = kd["flops"] / max(kd["bytes"], 1.0)
-        common = {"kernel": dom, "avg_launch_us": kd["avg_us"], "launches": kd["launches"],
-                  "algorithmic_bytes_per_launch": kd["bytes"] / kd["launches"],
-                  "algorithmic_flops_per_launch": kd["flops"] / kd["launches"],
-                  "flop_per_byte": intensity,
-                  "traffic": measured_traffic(dom, args.config)}
-        if intensity > FP64_PEAK_TFS / (HBM_PEAK_GBS / 1e3):
-            roof = {"bound": "mfma", "achieved": kd["TFs"], "peak": FP64_PEAK_TFS,
-                    "unit": "TFLOP/s", "frac": kd["TFs"] / FP64_PEAK_TFS, **common}
-        else:
-            roof = {"bound": "hbm", "achieved": kd["GBs"], "peak": HBM_PEAK_GBS,
-                    "unit": "GB/s", "frac": kd["GBs"] / HBM_PEAK_GBS, **common}
+    roof = roofline_of(hr["kernels"], args.config)
     cpu, why = (None, "disabled")
-    if args.cpu_baseline == "on" and args.config == "cfg2":
-        cpu, why = cpu_baseline(prob, hr["num_iter"], hr["nev_conv"], args.cpu_iters)
+    if args.cpu_baseline == "on" and args.config in ("cfg2", "cfg3"):
+        cpu, why = cpu_baseline(args.config, hr["num_iter"], hr["nev_conv"], args.cpu_iters)
     line = {
         "metric": "converged eigenpairs/s at 1e-8 residual, mixed vs FP64; p50/p99 solve latency",
         "value": value, "unit": "eigenpairs/s", "n_gpus": 1, "steps": args.steps,
@@ -561,6 +625,11 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
     }
+    if "fp64_cholqr" in results and "mixed" in results:
+        # the same orthogonalization on both sides: precision (and the
+        # autoshift that FP32 inner solves cannot use) alone
+        line["speedup_mixed_vs_fp64_cholqr"] = (np.mean(results["fp64_cholqr"]["seconds"]) /
+                                                np.mean(results["mixed"]["seconds"]))
     if "fp64" in results and "mixed" in results:
         line["speedup_mixed_vs_fp64"] = (np.mean(results["fp64"]["seconds"]) /
                                          np.mean(results["mixed"]["seconds"]))

@@ -171,6 +171,9 @@ EXPORTS = {
     "gcge_hip_device": (C.c_int, []),
     "gcge_hip_profile_enable": (None, [C.c_int]),
     "gcge_hip_profile_reset": (None, []),
+    "gcge_hip_profile_all": (None, [C.c_int]),
+    "gcge_hip_profile_read_all": (None, [C.c_int, C.POINTER(C.c_int64), C.POINTER(C.c_double),
+                                         C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "gcge_hip_profile_read": (None, [C.c_int, C.POINTER(C.c_int64), c_dbl_p, c_dbl_p,
                                      c_dbl_p]),
     "HIPGCG_EigenSolverSetup": (None, [C.c_double, C.c_int, C.c_int, C.c_int, c_dbl_p, C.c_int,

@@ -67,6 +67,12 @@ namespace {
 thread_local Ctx *t_ctx = nullptr;
 std::mutex g_ctx_mu;  // the registry of live contexts
 int g_live_ctx = 0;
+// Kernel timing of every thread: contexts created while it is on start
+// with their profiler on, and released contexts add their totals here.
+std::mutex g_prof_mu;
+bool g_prof_all = false;
+int64_t g_prof_launches[PROF_NKIND] = {0};
+double g_prof_ms[PROF_NKIND] = {0}, g_prof_bytes[PROF_NKIND] = {0}, g_prof_flops[PROF_NKIND] = {0};
 }
 
 Ctx &ctx()
@@ -75,6 +81,10 @@ Ctx &ctx()
         std::lock_guard<std::mutex> lk(g_ctx_mu);
         tune_load_default();
         Ctx *x = new Ctx();
+        {
+            std::lock_guard<std::mutex> lp(g_prof_mu);
+            x->prof.on = g_prof_all;
+        }
         HIPCHECK(hipGetDevice(&x->device));
         HIPCHECK(hipStreamCreateWithFlags(&x->stream, hipStreamNonBlocking));
         t_ctx = x;
@@ -231,6 +241,16 @@ void release_ctx()
     if (c == nullptr) return;
     c->sync();
     c->prof.collect();
+    {
+        std::lock_guard<std::mutex> lp(g_prof_mu);
+        if (g_prof_all)
+            for (int k = 0; k < PROF_NKIND; ++k) {
+                g_prof_launches[k] += c->prof.launches[k];
+                g_prof_ms[k] += c->prof.ms[k];
+                g_prof_bytes[k] += c->prof.bytes[k];
+                g_prof_flops[k] += c->prof.flops[k];
+            }
+    }
     if (c->dev_eig) release_dev_eig(c->dev_eig);
     if (c->d_scratch) dev_free(c->d_scratch);
     if (c->d_small) dev_free(c->d_small);
@@ -1059,6 +1079,29 @@ void gcge_hip_profile_enable(int on)
 }
 
 void gcge_hip_profile_reset(void) { ctx().prof.reset(); }
+
+void gcge_hip_profile_all(int on)
+{
+    std::lock_guard<std::mutex> lp(g_prof_mu);
+    g_prof_all = on != 0;
+    if (on)
+        for (int k = 0; k < PROF_NKIND; ++k) {
+            g_prof_launches[k] = 0;
+            g_prof_ms[k] = g_prof_bytes[k] = g_prof_flops[k] = 0.0;
+        }
+}
+
+void gcge_hip_profile_read_all(int kind, int64_t *launches, double *total_ms, double *bytes,
+                               double *flops)
+{
+    if (kind < 0 || kind >= PROF_NKIND)
+        fail("gcge_hip_profile_read_all: bad kind", __FILE__, __LINE__);
+    std::lock_guard<std::mutex> lp(g_prof_mu);
+    if (launches) *launches = g_prof_launches[kind];
+    if (total_ms) *total_ms = g_prof_ms[kind];
+    if (bytes) *bytes = g_prof_bytes[kind];
+    if (flops) *flops = g_prof_flops[kind];
+}
 
 void gcge_hip_profile_read(int kind, int64_t *launches, double *total_ms, double *bytes,
                            double *flops)

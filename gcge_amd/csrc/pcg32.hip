@@ -942,6 +942,7 @@ struct PcgBatcher {
     struct Job {
         BatchJob d;
         int it = 0, max_iter = 0, nt = 0;
+        double mat_bytes = 0.0, nnz = 0.0;  // of the job's matrix, for the kernel timing
         double *x64 = nullptr;
         int64_t ldx = 0;
         int64_t *iters = nullptr;
@@ -1013,9 +1014,30 @@ struct PcgBatcher {
             }
             HIPCHECK(hipMemcpyAsync(d_tab, h_tab, sizeof(BatchJob) * nj, hipMemcpyHostToDevice, st));
             for (int r = 0; r < nr; ++r) {
+                // algorithmic bytes and flops of the round, summed over the
+                // jobs it advances (the solo loop's formulas over each job's
+                // listed tiles)
+                double bp = 0.0, fp = 0.0, bs = 0.0, fs = 0.0, bx = 0.0, fx = 0.0;
+                for (int z = 0; z < nj; ++z) {
+                    const Job &j = *active[z];
+                    if (j.it + r >= j.max_iter) continue;
+                    const double tv = 4.0 * kC * (double)j.d.n * j.nt;
+                    bp += (j.it + r == 0 ? 2.0 : 3.0) * tv;
+                    fp += 2.0 * kC * (double)j.d.n * j.nt;
+                    bs += j.mat_bytes + 2.0 * tv;
+                    fs += 2.0 * j.nnz * kC * j.nt;
+                    bx += 6.0 * tv;
+                    fx += 6.0 * kC * (double)j.d.n * j.nt;
+                }
+                hipEvent_t e1 = c.prof.begin(PROF_PCGVEC, bp, fp, st);
                 k_batch_update_p<<<dim3(gx, gy, nj), kWG, 0, st>>>(d_tab, r);
+                c.prof.end(e1, st);
+                hipEvent_t e2 = c.prof.begin(PROF_SPMM32, bs, fs, st);
                 k_batch_spmm_pw<<<dim3(gr, 1, nj), kWG, 0, st>>>(d_tab, r);
+                c.prof.end(e2, st);
+                hipEvent_t e3 = c.prof.begin(PROF_PCGVEC, bx, fx, st);
                 k_batch_update_xr<<<dim3(gx, gy, nj), kWG, 0, st>>>(d_tab, r);
+                c.prof.end(e3, st);
             }
             k_batch_counts<<<1, kMaxJobs, 0, st>>>(d_tab, nj, nr, d_cnt);
             HIPCHECK(hipMemcpyAsync(h_cnt, d_cnt, sizeof(int) * nj, hipMemcpyDeviceToHost, st));
@@ -1147,7 +1169,7 @@ void pcg32_solve(void *matA, void **mv_b, void **mv_x, int *start, int *end, OPS
     // served solves of the fused kind (general rows of at most 8 entries,
     // no shift) run their iterations in the batcher's rounds
     if (S.batcher != nullptr && S.use_fused && n <= kFuseMaxRows && sparts == 0 && !two &&
-        sigma == 0.f && A->sell.max_width <= 8 && !c.prof.on) {
+        sigma == 0.f && A->sell.max_width <= 8) {
         PcgBatcher::Job j;
         BatchJob &d = j.d;
         d.n = n;
@@ -1176,6 +1198,8 @@ void pcg32_solve(void *matA, void **mv_b, void **mv_x, int *start, int *end, OPS
         d.nblk_row = (int)nblk_row;
         j.max_iter = S.max_iter;
         j.nt = ntact;
+        j.mat_bytes = (double)A->sell.nnz_true * 8.0 + 4.0 * (double)(n + 1);
+        j.nnz = (double)A->sell.nnz_true;
         j.x64 = Xv->d + (int64_t)start[1] * Xv->ldd;
         j.ldx = Xv->ldd;
         j.iters = &S.iters;

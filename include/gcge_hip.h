@@ -211,6 +211,13 @@ void gcge_hip_profile_enable(int on);
 void gcge_hip_profile_reset(void);
 void gcge_hip_profile_read(int kind, int64_t *launches, double *total_ms, double *bytes,
                            double *flops);
+/* The same timing for every thread (the solve service's workers and its
+ * batcher): with on != 0 the totals are cleared and contexts created
+ * afterwards time their launches; a context's totals are added when it is
+ * released (a worker's when the service is destroyed). */
+void gcge_hip_profile_all(int on);
+void gcge_hip_profile_read_all(int kind, int64_t *launches, double *total_ms, double *bytes,
+                               double *flops);
 
 /* Launch-shape tuner (no counterpart in the reference, whose kernels are
  * MKL's).  A process-wide table maps (operation, shape class) to a

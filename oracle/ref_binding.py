@@ -83,13 +83,14 @@ class RefLib:
         return {"eval": ev, "num_iter": int(st[0]), "nev_conv": int(st[1]),
                 "seconds": float(tm[0]), "evec": vec}
 
-    def solve_ccs(self, prob, shift: int = 1, max_iter: int = 1000, want_vectors=False):
+    def solve_ccs(self, prob, shift: int = 1, max_iter: int = 1000, want_vectors=False,
+                  print_level: int = 0):
         ops = self.ccs_table()
         hA, ka = self.wrap_ccs(prob.A)
         hB, kb = self.wrap_ccs(prob.B) if prob.B is not None else (None, None)
         out = self.solve(ops, hA, hB, prob.n, prob.nev, prob.block_size, prob.nev_init,
                          prob.nev_max, (prob.tol_abs, prob.tol_rel), prob.gap_min, shift,
-                         max_iter, want_vectors=want_vectors)
+                         max_iter, print_level=print_level, want_vectors=want_vectors)
         self.lib.ref_free(hA)
         if hB:
             self.lib.ref_free(hB)
