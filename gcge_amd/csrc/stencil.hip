@@ -290,7 +290,7 @@ __global__ __launch_bounds__(256) void k_stencil_pw(int n, int xw, int ygroups, 
     }
 }
 
-constexpr int kPwTY = 2;
+constexpr int kPwTY = 4;
 
 }  // namespace
 
@@ -320,7 +320,7 @@ bool spmm_stencil(const SellMat &A, const double *X, int64_t ldx, double *Y, int
                   int ncols, hipStream_t s)
 {
     if (A.stencil_n < 4 || A.stencil_n > kStencilMaxN || ncols <= 0) return false;
-    launch_stencil7<4>(A, X, ldx, Y, ldy, ncols, s);
+    launch_stencil7<8>(A, X, ldx, Y, ldy, ncols, s);
     return true;
 }
 

@@ -175,3 +175,36 @@ def test_cfg5_served_stream(hip):
     print(f"# 100 requests: max concurrent {st['max_concurrent']}, latency p50 "
           f"{np.percentile(lat, 50):.2f} s, p99 {np.percentile(lat, 99):.2f} s")
     assert st["max_concurrent"] >= 2
+
+
+@pytest.mark.timeout(900)
+def test_cfg2_reference_driver_over_hip_table(hip, reflib):
+    """The literal drop-in at the headline size: the reference's own GCG
+    driver (src/ops_eig_sol_gcg.c:1259-1562, compiled from its sources into
+    oracle/_ref) runs unchanged over the table OPS_HIP_Set fills, on the
+    128^3 problem with the plug-in's call sequence and srand(0), and
+    reproduces the reference's CPU run (tests/golden/ref_cfg2.json): the same
+    eigenvalues to 1e-9 and nearly the same iteration count."""
+    import ctypes as C
+    from gcge_amd import abi
+    gold = json.load(open(GOLD))["run"]
+    prob = P.config2_grid()
+    dA, dB = S.device_operator(prob.A), S.device_operator(prob.B)
+    ops = reflib.empty_table()
+    hip.OPS_HIP_Set(C.cast(C.c_void_p(ops), abi.OPS_p))
+    reflib.setup(ops)
+    hip.gcge_hip_set_random_mode(abi.GCGE_HIP_RAND_LIBC, 0)
+    r = reflib.solve(ops, dA.handle.value, dB.handle.value, 128 ** 3, prob.nev, prob.block_size,
+                     prob.nev_init, prob.nev_max, (prob.tol_abs, prob.tol_rel), prob.gap_min,
+                     1, 1000, host_vectors=False)
+    k = min(r["nev_conv"], gold["nev_conv"])
+    ref = np.array(gold["eval"][:k])
+    err = np.max(np.abs(r["eval"][:k] - ref) / ref)
+    exact = P.analytic_eigenvalues(128, 3, k)
+    err_a = np.max(np.abs(r["eval"][:k] - exact) / exact)
+    print(f"# reference driver over the HIP table: {r['seconds']:.2f} s, {r['num_iter']} "
+          f"iterations (CPU run {gold['num_iter']}), {r['nev_conv']} converged "
+          f"({gold['nev_conv']}), max rel diff {err:.2e}, vs analytic {err_a:.2e}")
+    assert r["nev_conv"] >= prob.nev
+    assert err <= 1e-9 and err_a <= 1e-10
+    assert abs(r["num_iter"] - gold["num_iter"]) <= 3
