@@ -48,10 +48,11 @@ bool spmm_stencil(const SellMat &A, const double *X, int64_t ldx, double *Y, int
                   int ncols, hipStream_t s);
 // The FP32 BlockPCG product w = A p over the listed 8-column tiles of its
 // interleaved blocks, matrix-free for a stencil operator (unshifted), with
-// partial p.w sums: stencil_pw_parts(A) per column (0 when A is not a
-// stencil operator), at partial[(tile 8 + t) parts + part].  The values
-// are the FP32 copies of the stencil's, as the matrix's FP32 copy holds.
-int stencil_pw_parts(const SellMat &A);
+// partial p.w sums: stencil_pw_parts(A, nt_bound) per column for a launch
+// over at most nt_bound tiles (0 when A is not a stencil operator), at
+// partial[(tile 8 + t) parts + part].  The values are the FP32 copies of
+// the stencil's, as the matrix's FP32 copy holds.
+int stencil_pw_parts(const SellMat &A, int nt_bound);
 void stencil_pw(const SellMat &A, const int *tiles, const int *ntact_p, int nt_bound,
                 const float *P, float *W, float *partial, hipStream_t s);
 // Y[:, c] = A X[:, c], c < ncols.

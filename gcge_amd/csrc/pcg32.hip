@@ -1164,8 +1164,13 @@ void pcg32_solve(void *matA, void **mv_b, void **mv_x, int *start, int *end, OPS
     // arrays of A (and of B when shifted) once, p read once, w written once
     // A constant-coefficient 7-point A without shift is applied matrix-free
     // (stencil.hip), with its own count of p.w partial sums per column
-    const int sparts = sigma == 0.f ? stencil_pw_parts(A->sell) : 0;
-    const int nparts_pw = sparts > 0 ? sparts : (int)nblk_row;
+    // (its most partial sums per column: those of a launch over one tile)
+    const int sp = sigma == 0.f ? stencil_pw_parts(A->sell, 1) : 0;
+    const int sparts = sp <= S.nparts_max() ? sp : 0;
+    // partial sums per column of a product launched over at most nt tiles
+    auto nparts_pw = [&](int nt) {
+        return sparts > 0 ? stencil_pw_parts(A->sell, nt) : (int)nblk_row;
+    };
     // served solves of the fused kind (general rows of at most 8 entries,
     // no shift) run their iterations in the batcher's rounds
     if (S.batcher != nullptr && S.use_fused && n <= kFuseMaxRows && sparts == 0 && !two &&
@@ -1298,11 +1303,11 @@ void pcg32_solve(void *matA, void **mv_b, void **mv_x, int *start, int *end, OPS
                               : nullptr;
         if (fused) {
             k_update_xr_pw<<<dim3(nb, nt), kWG, 0, st>>>(
-                n, tl, ntp, nparts_pw, S.partial, rho_buf[par], act_buf[par], S.p, S.w, S.x,
+                n, tl, ntp, nparts_pw(nt), S.partial, rho_buf[par], act_buf[par], S.p, S.w, S.x,
                 S.r, S.partial_rr);
             if (prof) c.prof.end(evx, st);
         } else {
-            k_finish_pw<<<nt * kC, 64, 0, st>>>(S.tiles, S.ntact, nparts_pw, S.partial,
+            k_finish_pw<<<nt * kC, 64, 0, st>>>(S.tiles, S.ntact, nparts_pw(nt), S.partial,
                                                 S.rho, S.active, S.alpha);
             k_update_xr<<<dim3(nb, nt), kWG, 0, st>>>(n, S.tiles, S.ntact, S.alpha, S.w, S.r,
                                                       S.partial);

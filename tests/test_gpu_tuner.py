@@ -1,5 +1,5 @@
 """The launch-shape tuner on the GPU: every SpMM launch shape gives the
-same bits; a solve with tuning on records its choices, and the saved table
+same bits (and so does the matrix-free 7-point kernel); a solve with tuning on records its choices, and the saved table
 reproduces that solve bit for bit."""
 import ctypes as C
 
@@ -77,6 +77,17 @@ def test_spmm_shapes_give_identical_bits(lib, kind):
                                           (C.c_int * 2)(k, k), table.ops)
             outs.append(table.download(hy, 0, k, n))
             table.destroy_multivec(hy, k)
+        if kind == "7pt":
+            # and the matrix-free stencil kernel that the same matrix gets
+            # when it is recognised (no tuned shapes of its own)
+            dS = DeviceMatrix(A)
+            assert dS.stencil() == 30
+            hy = table.create_multivec(k, dS)
+            table.slot("MatDotMultiVec")(dS.handle, hx, hy, (C.c_int * 2)(0, 0),
+                                          (C.c_int * 2)(k, k), table.ops)
+            outs.append(table.download(hy, 0, k, n))
+            table.destroy_multivec(hy, k)
+            dS.free()
         table.destroy_multivec(hx, k)
         print(f"spmm {kind} N={n} k={k}: {ncand} shapes", flush=True)
         for o in outs[1:]:
