@@ -13,6 +13,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstdlib>
+#include <climits>
 #include <cstring>
 #include <iterator>
 #include <map>
@@ -23,6 +24,7 @@
 
 #include "app_hip_internal.h"
 #include "gcge_hip.h"
+#include "dense_eig.h"
 #include "kernels.h"
 #include "tuner.h"
 
@@ -552,6 +554,43 @@ void h_MatDotMultiVec(void *mat, void **x, void **y, int *start, int *end, OPS *
     c.stats.spmm_cols += nc;
 }
 
+// r[:, sr:sr+nc] = A x[:, sx:sx+nc] - B x[:, sx:sx+nc] diag(lam) in one pass
+// when ops is this table and A, B are stencil operators of one grid; the
+// same bits as MatDotMultiVec by A and by B, MultiVecLinearComb scaling by
+// lam and MultiVecAxpby(-1, ., 1, .), the residual sequence of GCG's
+// convergence check (src/ops_eig_sol_gcg.c:210-222).  False (nothing done)
+// otherwise.
+bool stencil_residual_slot(OPS *ops, void *matA, void *matB, void **x, int sx, int nc,
+                           const double *lam, void **r, int sr)
+{
+    if (ops == nullptr || ops->MatDotMultiVec != h_MatDotMultiVec || matA == nullptr ||
+        matB == nullptr || nc <= 0)
+        return false;
+    Mat *A = mat_of(matA, "stencil residual A"), *B = mat_of(matB, "stencil residual B");
+    if (A->sell.stencil_n == 0 || B->sell.stencil_n != A->sell.stencil_n) return false;
+    MV *X = mv_of(x, "stencil residual x"), *R = mv_of(r, "stencil residual r");
+    check_range(X, sx, sx + nc, "stencil residual x");
+    check_range(R, sr, sr + nc, "stencil residual r");
+    if (A->n != X->nrows || A->n != R->nrows || B->n != A->n)
+        fail("stencil residual: matrix and vector sizes differ", __FILE__, __LINE__);
+    Ctx &c = ctx();
+    Ctx::Staging &g = c.staging(nc);
+    for (int k = 0; k < nc; ++k) g.h[k] = lam[k];
+    HIPCHECK(hipMemcpyAsync(g.d, g.h, sizeof(double) * nc, hipMemcpyHostToDevice, c.stream));
+    // X read once, R written once; two products' flops
+    hipEvent_t ev = c.prof.begin(PROF_SPMM64, 2.0 * 8.0 * (double)nc * (double)A->n,
+                                 2.0 * (double)(A->sell.nnz_true + B->sell.nnz_true) * nc,
+                                 c.stream);
+    const bool ok = stencil_residual(A->sell, B->sell, X->d + (int64_t)sx * X->ldd, X->ldd, g.d,
+                                     R->d + (int64_t)sr * R->ldd, R->ldd, nc, c.stream);
+    c.prof.end(ev, c.stream);
+    c.staged(g);
+    if (!ok) fail("stencil residual: operators not handled", __FILE__, __LINE__);
+    c.stats.spmm_calls++;
+    c.stats.spmm_cols += nc;
+    return true;
+}
+
 // y = alpha x + beta y over column ranges (app/app_lapack.c:334-395)
 void h_MultiVecAxpby(double alpha, void **x, double beta, void **y, int *start, int *end,
                      OPS *)
@@ -602,14 +641,26 @@ void h_MultiVecLinearComb(void **x, void **y, int, int *start, int *end, double 
     MV *X = mv_of(x, "MultiVecLinearComb x");
     check_range(X, start[0], end[0], "MultiVecLinearComb x");
     if (X->nrows != Y->nrows) fail("MultiVecLinearComb: row counts differ", __FILE__, __LINE__);
-    // stage coef transposed and k-major (Ct[i * nc + k] = coef[i + k * ldc]),
-    // the operand layout of the matrix-core kernel, and beta, in one upload
+    // Over a million rows with at least 100 source columns and no
+    // per-column beta the library GEMM is faster (X C + beta Y is a plain
+    // GEMM; 61-66 TFLOP/s against 48-53 for the matrix-core kernel at 2 M
+    // rows, 120-460 source and 120-220 destination columns); otherwise the
+    // matrix-core kernel (equal at 262,144 rows, faster below and for
+    // narrow blocks).  The library takes coef as it is (column-major, nr
+    // rows); the kernel transposed and k-major (Ct[i * nc + k] = coef[i + k
+    // * ldc]).  Coefficients and beta go in one upload.
+    const bool library = Y->nrows >= (int64_t(1) << 20) && nr >= 100 &&
+                         (beta == nullptr || incb == 0) && Y->nrows <= INT32_MAX &&
+                         X->ldd <= INT32_MAX && Y->ldd <= INT32_MAX;
     const int64_t ncoef = (int64_t)nr * nc;
     Ctx::Staging &g = c.staging(ncoef + nc);
     double *h = g.h;
     for (int k = 0; k < nc; ++k) {
         const double *src = coef + (int64_t)ldc * k;
-        for (int i = 0; i < nr; ++i) h[(int64_t)i * nc + k] = src[i];
+        if (library)
+            memcpy(h + (int64_t)nr * k, src, sizeof(double) * nr);
+        else
+            for (int i = 0; i < nr; ++i) h[(int64_t)i * nc + k] = src[i];
     }
     int beta_mode = 0;
     double beta_scalar = 0.0;
@@ -628,8 +679,12 @@ void h_MultiVecLinearComb(void **x, void **y, int, int *start, int *end, double 
     hipEvent_t ev = c.prof.begin(
         PROF_LINCOMB, 8.0 * (double)Y->nrows * (nr + nc * (beta_mode ? 2.0 : 1.0)),
         2.0 * (double)Y->nrows * nr * nc, c.stream);
-    lincomb(Y->nrows, nr, nc, X->d + (int64_t)start[0] * X->ldd, X->ldd, d, nc, beta_mode,
-            beta_scalar, d + ncoef, ys, Y->ldd, c.stream);
+    if (library)
+        gemm_nn(Y->nrows, nc, nr, X->d + (int64_t)start[0] * X->ldd, X->ldd, d, nr,
+                beta_mode == 1 ? beta_scalar : 0.0, ys, Y->ldd);
+    else
+        lincomb(Y->nrows, nr, nc, X->d + (int64_t)start[0] * X->ldd, X->ldd, d, nc, beta_mode,
+                beta_scalar, d + ncoef, ys, Y->ldd, c.stream);
     c.prof.end(ev, c.stream);
     c.stats.lincomb_calls++;
     c.stats.lincomb_flops += 2.0 * (double)Y->nrows * nr * nc;
@@ -791,6 +846,14 @@ void h_MatDotVec(void *m, void *x, void *y, OPS *ops)
 }
 
 }  // namespace
+
+namespace gcge {
+bool hip_stencil_residual(OPS *ops, void *A, void *B, void **x, int sx, int nc,
+                          const double *lam, void **r, int sr)
+{
+    return stencil_residual_slot(ops, A, B, x, sx, nc, lam, r, sr);
+}
+}  // namespace gcge
 
 // ------------------------------------------------------------------ C ABI
 extern "C" {

@@ -11,6 +11,7 @@
 #include <rocsolver/rocsolver.h>
 
 #include <algorithm>
+#include <cstdint>
 #include <cmath>
 #include <cstring>
 #include <numeric>
@@ -204,6 +205,19 @@ void release_dev_eig(void *p)
     free_buffers(*d);
     rocblas_destroy_handle(d->h);
     delete d;
+}
+
+void gemm_nn(int64_t m, int n, int k, const double *A, int64_t lda, const double *B, int ldb,
+             double beta, double *C, int64_t ldc)
+{
+    if (m > INT32_MAX || lda > INT32_MAX || ldc > INT32_MAX)
+        fail("gemm_nn: dimensions past the library's 32-bit range", __FILE__, __LINE__);
+    DevEig &d = deveig();
+    const double one = 1.0;
+    const rocblas_status st =
+        rocblas_dgemm(d.h, rocblas_operation_none, rocblas_operation_none, (rocblas_int)m, n, k,
+                      &one, A, (rocblas_int)lda, B, ldb, &beta, C, (rocblas_int)ldc);
+    if (st != rocblas_status_success) fail("rocblas_dgemm failed", __FILE__, __LINE__);
 }
 
 int eigh_device(int n, double *A, int lda, double *w, double *Z, int ldz)

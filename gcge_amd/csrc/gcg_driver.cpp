@@ -31,6 +31,11 @@ namespace gcge {
 // Inner iterations the device FP32 BlockPCG of `ops` has run since it was
 // installed (0 when the slot holds another solver); pcg32.hip.
 int64_t pcg32_iterations(OPS *ops);
+// r[:, sr:] = A x[:, sx:] - B x[:, sx:] diag(lam) over nc columns in one
+// matrix-free pass when ops is the HIP table and A, B are 7-point stencil
+// operators of one grid (false otherwise); app_hip.cpp.
+bool hip_stencil_residual(OPS *ops, void *A, void *B, void **x, int sx, int nc,
+                          const double *lam, void **r, int sr);
 }
 
 namespace {
@@ -514,13 +519,18 @@ int check_convergence(Solver &S, void **ritz, int num_check, std::vector<int> &o
     OPS *ops = S.ops;
     void **ws0 = S.mv_ws[1], **ws1 = S.mv_ws[2];
     int s[2] = {S.startN, 0}, e[2] = {S.startN + num_check, num_check};
-    ops->MatDotMultiVec(S.A, ritz, ws0, s, e, ops);
-    ops->MatDotMultiVec(S.B, ritz, ws1, s, e, ops);
-    ops->MultiVecLinearComb(nullptr, ws1, 0, s, e, nullptr, 0, S.ss_eval.data() + S.startN, 1,
-                            ops);
+    if (!gcge::hip_stencil_residual(ops, S.A, S.B, ritz, S.startN, num_check,
+                                    S.ss_eval.data() + S.startN, ws0, 0)) {
+        ops->MatDotMultiVec(S.A, ritz, ws0, s, e, ops);
+        ops->MatDotMultiVec(S.B, ritz, ws1, s, e, ops);
+        ops->MultiVecLinearComb(nullptr, ws1, 0, s, e, nullptr, 0, S.ss_eval.data() + S.startN,
+                                1, ops);
+        s[0] = 0; e[0] = num_check;
+        s[1] = 0; e[1] = num_check;
+        ops->MultiVecAxpby(-1.0, ws1, 1.0, ws0, s, e, ops);
+    }
     s[0] = 0; e[0] = num_check;
     s[1] = 0; e[1] = num_check;
-    ops->MultiVecAxpby(-1.0, ws1, 1.0, ws0, s, e, ops);
     std::vector<double> res((size_t)num_check);
     ops->MultiVecInnerProd('D', ws0, ws0, 0, s, e, res.data(), 1, ops);
     for (auto &v : res) v = std::sqrt(v);

@@ -46,6 +46,13 @@ constexpr int kStencilMaxN = 128;
 // same bits as the SELL kernels).
 bool spmm_stencil(const SellMat &A, const double *X, int64_t ldx, double *Y, int64_t ldy,
                   int ncols, hipStream_t s);
+// R[:, c] = A X[:, c] - lam[c] B X[:, c] (lam on the device) when A and B
+// are constant-coefficient 7-point operators on the same grid that the
+// stencil kernels handle (false otherwise): one pass over X, and the bits
+// that spmm by A and by B, a scaling of the second by lam (scale_cols) and
+// their difference (axpby) give.
+bool stencil_residual(const SellMat &A, const SellMat &B, const double *X, int64_t ldx,
+                      const double *lam, double *R, int64_t ldr, int ncols, hipStream_t s);
 // The FP32 BlockPCG product w = A p over the listed 8-column tiles of its
 // interleaved blocks, matrix-free for a stencil operator (unshifted), with
 // partial p.w sums: stencil_pw_parts(A, nt_bound) per column for a launch

@@ -53,9 +53,22 @@ __device__ __forceinline__ double lane_down(double v)
 // of plane z + 1) before it computes plane z, so they are in flight during
 // the computation and the exchange; four plane register sets rotate by
 // unrolling the march by four, so no register move waits on a load.
-template <int kTY, int kW, bool VEC>
+// RESID: Y[:, c] = A X[:, c] - lam[c] (B X[:, c]) for a second operator B =
+// dB I + oB Adj on the same grid, each product summed as above, then the
+// scaling and the difference rounded separately -- the bits of the product
+// by A, the product by B scaled by lam[c] (0 when lam[c] is 0) and the
+// difference, as three separate kernels give them.
+__device__ __forceinline__ double resid_of(double ax, double bx, double lam)
+{
+#pragma clang fp contract(off)
+    const double t = lam == 0.0 ? 0.0 : lam * bx;
+    return ax - t;
+}
+
+template <int kTY, int kW, bool VEC, bool RESID>
 __global__ __launch_bounds__(64 * kW) void k_stencil7(int n, int ygroups, int zchunks, int zc,
-                                                   double d, double o,
+                                                   double d, double o, double dB, double oB,
+                                                   const double *__restrict__ lam,
                                                    const double *__restrict__ X, int64_t ldx,
                                                    double *__restrict__ Y, int64_t ldy)
 {
@@ -77,6 +90,7 @@ __global__ __launch_bounds__(64 * kW) void k_stencil7(int n, int ygroups, int zc
     const int64_t nn = (int64_t)n * n;
     const double *Xc = X + c * ldx;
     double *Yc = Y + c * ldy;
+    const double lam_c = RESID ? lam[c] : 0.0;
     // the outer halo line of this wavefront (wavefronts 0 and 3 only)
     const int yh = wv == 0 ? y0 - 1 : y0 + kTY;
     const bool outer = wave_on && (wv == 0 || wv == kW - 1);
@@ -157,6 +171,17 @@ __global__ __launch_bounds__(64 * kW) void k_stencil7(int n, int ygroups, int zc
                 if (y < n - 1) acc = __builtin_fma(o, yp, acc);
                 if (z < n - 1) acc = __builtin_fma(o, pn[i][p], acc);
                 out[p] = acc;
+                if constexpr (RESID) {
+                    double bcc = 0.0;
+                    if (z > 0) bcc = __builtin_fma(oB, pm[i][p], bcc);
+                    if (y > 0) bcc = __builtin_fma(oB, ym, bcc);
+                    if (x > 0) bcc = __builtin_fma(oB, xm, bcc);
+                    bcc = __builtin_fma(dB, pc[i][p], bcc);
+                    if (x < n - 1) bcc = __builtin_fma(oB, xp, bcc);
+                    if (y < n - 1) bcc = __builtin_fma(oB, yp, bcc);
+                    if (z < n - 1) bcc = __builtin_fma(oB, pn[i][p], bcc);
+                    out[p] = resid_of(acc, bcc, lam_c);
+                }
             }
             const int64_t r = x0 + (int64_t)n * y + nn * z;
             if constexpr (VEC) {
@@ -187,7 +212,8 @@ __global__ __launch_bounds__(64 * kW) void k_stencil7(int n, int ygroups, int zc
 
 template <int kTY, int kW>
 void launch_stencil7(const SellMat &A, const double *X, int64_t ldx, double *Y, int64_t ldy,
-                      int ncols, int zc, hipStream_t s)
+                     int ncols, int zc, hipStream_t s, const SellMat *B = nullptr,
+                     const double *lam = nullptr)
 {
     const int n = A.stencil_n;
     const int ygroups = (n + kW * kTY - 1) / (kW * kTY);
@@ -196,12 +222,16 @@ void launch_stencil7(const SellMat &A, const double *X, int64_t ldx, double *Y, 
     const bool vec = (n % 2 == 0) && (ldx % 2 == 0) && (ldy % 2 == 0) &&
                      (reinterpret_cast<uintptr_t>(X) % 16 == 0) &&
                      (reinterpret_cast<uintptr_t>(Y) % 16 == 0);
-    if (vec)
-        k_stencil7<kTY, kW, true><<<(unsigned)blocks, 64 * kW, 0, s>>>(
-            n, ygroups, zchunks, zc, A.stencil_d, A.stencil_o, X, ldx, Y, ldy);
-    else
-        k_stencil7<kTY, kW, false><<<(unsigned)blocks, 64 * kW, 0, s>>>(
-            n, ygroups, zchunks, zc, A.stencil_d, A.stencil_o, X, ldx, Y, ldy);
+    const double dB = B ? B->stencil_d : 0.0, oB = B ? B->stencil_o : 0.0;
+#define GCGE_ST7(V, R)                                                                      \
+    k_stencil7<kTY, kW, V, R><<<(unsigned)blocks, 64 * kW, 0, s>>>(                         \
+        n, ygroups, zchunks, zc, A.stencil_d, A.stencil_o, dB, oB, lam, X, ldx, Y, ldy)
+    if (B) {
+        if (vec) GCGE_ST7(true, true); else GCGE_ST7(false, true);
+    } else {
+        if (vec) GCGE_ST7(true, false); else GCGE_ST7(false, false);
+    }
+#undef GCGE_ST7
 }
 // ---------------------------------------------------------------------
 // The FP32 BlockPCG product w = A p over its 8-column tiles (element
@@ -438,6 +468,16 @@ bool spmm_stencil(const SellMat &A, const double *X, int64_t ldx, double *Y, int
 {
     if (A.stencil_n < 4 || A.stencil_n > kStencilMaxN || ncols <= 0) return false;
     launch_stencil7<k7TY, k7W>(A, X, ldx, Y, ldy, ncols, k7Zc, s);
+    return true;
+}
+
+bool stencil_residual(const SellMat &A, const SellMat &B, const double *X, int64_t ldx,
+                      const double *lam, double *R, int64_t ldr, int ncols, hipStream_t s)
+{
+    if (A.stencil_n < 4 || A.stencil_n > kStencilMaxN || B.stencil_n != A.stencil_n ||
+        ncols <= 0)
+        return false;
+    launch_stencil7<k7TY, k7W>(A, X, ldx, R, ldr, ncols, k7Zc, s, &B, lam);
     return true;
 }
 
