@@ -191,7 +191,12 @@ def rocprof_summary(d: str):
 
 # kernel class of the library profiler -> device kernel name prefix
 # (one profiler launch of these classes is one dispatch of one of the kernels)
-KERNEL_OF_CLASS = {"lincomb": ["k_lincomb_mfma"], "spmm64": ["k_spmm_sell", "k_stencil7"],
+# the library GEMM of the large LinearComb calls (rocBLAS's Tensile FP64 NN
+# kernel that it picks for 2 M rows on MI355X; a trailing '*' matches a
+# prefix of the full kernel name)
+GEMM_KERNEL = "Cijk_Ailk_Bljk_DB_MT128x128x8_MI16x16x4x1*"
+KERNEL_OF_CLASS = {"lincomb": ["k_lincomb_mfma"], "gemm": [GEMM_KERNEL],
+                   "spmm64": ["k_spmm_sell", "k_stencil7"],
                    "spmm32": ["k_spmm_pw", "k_stencil_pw"],
                    "pcgvec": ["k_update_p", "k_update_xr"]}
 
@@ -215,11 +220,16 @@ def _short(kernel: str) -> str:
     return k.split("(")[0].split("::")[-1]
 
 
+def _match(kernel: str, pf: str) -> bool:
+    if pf.endswith("*"):
+        return _short(kernel).startswith(pf[:-1])
+    return _short(kernel).split("<")[0] == pf.split("<")[0] and (
+        "<" not in pf or _short(kernel) == pf)
+
+
 def _find(means: dict, prefix):
     prefixes = [prefix] if isinstance(prefix, str) else prefix
-    hits = [(k, v) for k, v in means.items() for pf in prefixes
-            if _short(k).split("<")[0] == pf.split("<")[0]
-            and (("<" not in pf) or _short(k) == pf)]
+    hits = [(k, v) for k, v in means.items() for pf in prefixes if _match(k, pf)]
     if not hits:
         return None
     # several template instances: dispatch-weighted mean
@@ -266,7 +276,7 @@ def pmc_traffic(fetch: str, write: str, cfetch: str, cwrite: str, bench_json: st
 
 
 # kernel class of the library profiler -> its MFMA kernel
-MFMA_CLASSES = {"gram": "k_gram_mfma", "lincomb": "k_lincomb_mfma"}
+MFMA_CLASSES = {"gram": "k_gram_mfma", "lincomb": "k_lincomb_mfma", "gemm": GEMM_KERNEL}
 
 
 def pmc_mfma(summary: str, bench_json: str):
@@ -383,7 +393,7 @@ def cpu_baseline_served(config, count):
                        f"({r['num_iter']} outer iterations)")}, None
 
 
-PROF_KINDS = ("spmm64", "spmm32", "gram", "lincomb", "blas1", "pcgvec")
+PROF_KINDS = ("spmm64", "spmm32", "gram", "lincomb", "blas1", "pcgvec", "gemm")
 
 
 def roofline_of(kern, config):

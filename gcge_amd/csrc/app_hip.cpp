@@ -677,7 +677,8 @@ void h_MultiVecLinearComb(void **x, void **y, int, int *start, int *end, double 
     HIPCHECK(hipMemcpyAsync(d, h, sizeof(double) * (ncoef + (beta_mode == 2 ? nc : 0)),
                             hipMemcpyHostToDevice, c.stream));
     hipEvent_t ev = c.prof.begin(
-        PROF_LINCOMB, 8.0 * (double)Y->nrows * (nr + nc * (beta_mode ? 2.0 : 1.0)),
+        library ? PROF_GEMM : PROF_LINCOMB,
+        8.0 * (double)Y->nrows * (nr + nc * (beta_mode ? 2.0 : 1.0)),
         2.0 * (double)Y->nrows * nr * nc, c.stream);
     if (library)
         gemm_nn(Y->nrows, nc, nr, X->d + (int64_t)start[0] * X->ldd, X->ldd, d, nr,

@@ -49,7 +49,7 @@ struct Stats {
 // Per-kernel-class timing with HIP events recorded on the library stream
 // around each launch; elapsed times are collected when read.
 enum ProfKind { PROF_SPMM64 = 0, PROF_SPMM32 = 1, PROF_GRAM = 2, PROF_LINCOMB = 3,
-                PROF_BLAS1 = 4, PROF_PCGVEC = 5, PROF_NKIND = 6 };
+                PROF_BLAS1 = 4, PROF_PCGVEC = 5, PROF_GEMM = 6, PROF_NKIND = 7 };
 struct Profiler {
     bool on = false;
     struct Rec {

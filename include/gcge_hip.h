@@ -205,8 +205,9 @@ int gcge_hip_device(void);
  * and every result written once).
  *   kind 0: FP64 SpMM (MatDotMultiVec), 1: FP32 shifted SpMM of the
  *   device BlockPCG, 2: Gram (inner products, both kernels),
- *   3: LinearComb, 4: BLAS-1, 5: vector updates of the device BlockPCG
- *   (p = r + beta p; x += alpha p, r -= alpha w with r.r). */
+ *   3: LinearComb (matrix-core kernel), 4: BLAS-1, 5: vector updates of
+ *   the device BlockPCG (p = r + beta p; x += alpha p, r -= alpha w with
+ *   r.r), 6: LinearComb by the library GEMM (over a million rows). */
 void gcge_hip_profile_enable(int on);
 void gcge_hip_profile_reset(void);
 void gcge_hip_profile_read(int kind, int64_t *launches, double *total_ms, double *bytes,
