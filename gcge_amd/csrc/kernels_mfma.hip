@@ -35,157 +35,198 @@ __device__ __forceinline__ d4 mfma(double a, double b, d4 c)
 }
 
 // ---------------------------------------------------------------------
-// Gram: G (m x n) = X^T Y.  A workgroup owns a 128 x 128 tile of G over one
-// chunk of rows; its four waves own 64 x 64 quadrants (4 x 4 MFMA tiles).
+// Gram: G (m x n) = X^T Y.  A workgroup owns a TM x TN tile of G over one
+// chunk of rows; its four waves, WM along m and 4 / WM along n, own 64 x
+// 16 NB blocks (4 x NB MFMA tiles).  The default shape is 128 x 128 (WM 2,
+// NB 4); a Y side of at most 64 columns -- the block width of most
+// orthogonalization Grams -- takes 256 x 16 NB tiles (WM 4, NB = its
+// 16-column tiles), and both sides of at most 64 a 64 x 64 tile (WM 1, NB
+// 1), so the matrix cores do not multiply 128-wide zero padding.
 // Rows are staged 16 at a time through LDS, stored column-major (row index
 // fastest, padded to 17) so each 16-lane MFMA operand read covers the
 // banks once.  Each thread fetches 4 consecutive rows (one 32-byte load)
-// of 4 columns per operand, and the next stage's loads are issued into
-// registers before the current stage's MFMAs, so HBM latency overlaps the
-// matrix-core work; at 98 VGPRs + 128 AGPRs two waves share each SIMD.
-// Partial tiles per chunk are summed by k_gram_mfma_reduce in a fixed
-// order (deterministic, no atomics).
+// of its columns of each operand, and the next stage's loads are issued
+// into registers before the current stage's MFMAs, so HBM latency overlaps
+// the matrix-core work; the default shape holds 98 VGPRs + 128 AGPRs, two
+// waves per SIMD.  Partial tiles per chunk are summed by
+// k_gram_mfma_reduce in a fixed order (deterministic, no atomics).  Each
+// entry sums its rows in the same order whatever the shape; the chunking,
+// and so the rounding of the partial sums, follows the tile count.
 // ---------------------------------------------------------------------
-constexpr int kGT = 128;
-constexpr int kGK = 16;                 // rows per stage
+constexpr int kGK = 16;        // rows per stage
 constexpr int kGKP = kGK + 1;
-constexpr int kGRG = kGK / 4;           // 4-row groups per stage
-constexpr int kGCT = kGT * kGRG / kWG;  // columns per thread per operand
+constexpr int kGRG = kGK / 4;  // 4-row groups per stage
 
+template <int WM, int NB>
 __global__ __launch_bounds__(kWG) void k_gram_mfma(
     int64_t nrows, int m, int n, const double *__restrict__ X, int64_t ldx,
     const double *__restrict__ Y, int64_t ldy, int tm, int tn, int64_t rows_per_chunk,
     int lower_only, double *__restrict__ partial)
 {
-    __shared__ double sx[kGT * kGKP];
-    __shared__ double sy[kGT * kGKP];
+    constexpr int WN = 4 / WM;
+    constexpr int TM = WM * 64, TN = WN * 16 * NB;
+    constexpr int XC = TM * kGRG / kWG;                  // X columns per thread
+    constexpr int YC = (TN * kGRG + kWG - 1) / kWG;      // Y columns per thread
+    __shared__ double sx[TM * kGKP];
+    __shared__ double sy[TN * kGKP];
     const int64_t lb = xcd_swizzle(blockIdx.x, gridDim.x);
     const int tiles = tm * tn;
     const int chunk = (int)(lb / tiles);
     const int t = (int)(lb % tiles);
     const int bi = t / tn, bj = t % tn;
-    double *out = partial + ((int64_t)chunk * tiles + t) * (kGT * kGT);
-    if (lower_only && bi < bj) {
-        for (int e = threadIdx.x; e < kGT * kGT; e += kWG) out[e] = 0.0;
+    double *out = partial + ((int64_t)chunk * tiles + t) * (TM * TN);
+    const int i0 = bi * TM, j0 = bj * TN;
+    if (lower_only && i0 + TM - 1 < j0) {  // wholly above the diagonal
+        for (int e = threadIdx.x; e < TM * TN; e += kWG) out[e] = 0.0;
         return;
     }
-    const int i0 = bi * kGT, j0 = bj * kGT;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int wm = wave >> 1, wn = wave & 1;
+    const int wm = wave / WN, wn = wave % WN;
     const int li = lane & 15, lk = lane >> 4;
     const int64_t rbeg = (int64_t)chunk * rows_per_chunk;
     int64_t rend = rbeg + rows_per_chunk;
     if (rend > nrows) rend = nrows;
-    // this thread's slots in a stage: rows 4*rg .. 4*rg+3 of columns
-    // kGCT*cg .. kGCT*cg+kGCT-1 (one 32-byte load per column; kGRG lanes
-    // cover the stage's rows of a column, so a wave instruction reads whole
-    // row segments of 64/kGRG columns, and only kGCT column addresses per
-    // operand stay live)
+    // this thread's slots in a stage: rows 4*rg .. 4*rg+3 of its columns
+    // (one 32-byte load per column; kGRG lanes cover the stage's rows of a
+    // column, so a wave instruction reads whole row segments of 64/kGRG
+    // columns, and only the thread's column addresses stay live)
     const int rg = threadIdx.x % kGRG;
     const int cg = threadIdx.x / kGRG;
-    const double *xc[kGCT], *yc[kGCT];
-    bool xok[kGCT], yok[kGCT];
+    const double *xc[XC], *yc[YC];
+    bool xok[XC], yok[YC];
 #pragma unroll
-    for (int c = 0; c < kGCT; ++c) {
-        const int cc = cg * kGCT + c;
+    for (int c = 0; c < XC; ++c) {
+        const int cc = cg * XC + c;
         xok[c] = i0 + cc < m;
-        yok[c] = j0 + cc < n;
         xc[c] = X + (int64_t)(xok[c] ? i0 + cc : 0) * ldx + 4 * rg;
+    }
+#pragma unroll
+    for (int c = 0; c < YC; ++c) {
+        const int cc = cg * YC + c;
+        yok[c] = cc < TN && j0 + cc < n;
         yc[c] = Y + (int64_t)(yok[c] ? j0 + cc : 0) * ldy + 4 * rg;
     }
-    double px[kGCT][4], py[kGCT][4];
+    double px[XC][4], py[YC][4];
     auto fetch = [&](int64_t r0) {
         const int64_t r = r0 + 4 * rg;
         if (r + 4 <= rend) {
 #pragma unroll
-            for (int c = 0; c < kGCT; ++c) {
-                const d4 vx = xok[c] ? *reinterpret_cast<const d4 *>(xc[c] + r0)
-                                     : d4{0.0, 0.0, 0.0, 0.0};
-                const d4 vy = yok[c] ? *reinterpret_cast<const d4 *>(yc[c] + r0)
-                                     : d4{0.0, 0.0, 0.0, 0.0};
+            for (int c = 0; c < XC; ++c) {
+                const d4 v = xok[c] ? *reinterpret_cast<const d4 *>(xc[c] + r0)
+                                    : d4{0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-                for (int t = 0; t < 4; ++t) {
-                    px[c][t] = vx[t];
-                    py[c][t] = vy[t];
-                }
+                for (int q = 0; q < 4; ++q) px[c][q] = v[q];
+            }
+#pragma unroll
+            for (int c = 0; c < YC; ++c) {
+                const d4 v = yok[c] ? *reinterpret_cast<const d4 *>(yc[c] + r0)
+                                    : d4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+                for (int q = 0; q < 4; ++q) py[c][q] = v[q];
             }
         } else {
 #pragma unroll
-            for (int c = 0; c < kGCT; ++c)
+            for (int q = 0; q < 4; ++q) {
+                const bool rok = r + q < rend;
 #pragma unroll
-                for (int t = 0; t < 4; ++t) {
-                    const bool rok = r + t < rend;
-                    px[c][t] = (rok && xok[c]) ? xc[c][r0 + t] : 0.0;
-                    py[c][t] = (rok && yok[c]) ? yc[c][r0 + t] : 0.0;
-                }
+                for (int c = 0; c < XC; ++c) px[c][q] = (rok && xok[c]) ? xc[c][r0 + q] : 0.0;
+#pragma unroll
+                for (int c = 0; c < YC; ++c) py[c][q] = (rok && yok[c]) ? yc[c][r0 + q] : 0.0;
+            }
         }
     };
-    d4 acc[4][4];
+    d4 acc[4][NB];
 #pragma unroll
     for (int a = 0; a < 4; ++a)
 #pragma unroll
-        for (int b = 0; b < 4; ++b) acc[a][b] = d4{0.0, 0.0, 0.0, 0.0};
+        for (int b = 0; b < NB; ++b) acc[a][b] = d4{0.0, 0.0, 0.0, 0.0};
     if (rbeg < rend) fetch(rbeg);
     for (int64_t r0 = rbeg; r0 < rend; r0 += kGK) {
 #pragma unroll
-        for (int c = 0; c < kGCT; ++c)
+        for (int c = 0; c < XC; ++c)
 #pragma unroll
-            for (int t = 0; t < 4; ++t) {
-                sx[(cg * kGCT + c) * kGKP + 4 * rg + t] = px[c][t];
-                sy[(cg * kGCT + c) * kGKP + 4 * rg + t] = py[c][t];
+            for (int q = 0; q < 4; ++q) sx[(cg * XC + c) * kGKP + 4 * rg + q] = px[c][q];
+#pragma unroll
+        for (int c = 0; c < YC; ++c) {
+            if (cg * YC + c < TN) {
+#pragma unroll
+                for (int q = 0; q < 4; ++q) sy[(cg * YC + c) * kGKP + 4 * rg + q] = py[c][q];
             }
+        }
         __syncthreads();
         if (r0 + kGK < rend) fetch(r0 + kGK);
 #pragma unroll 2
         for (int kk = 0; kk < kGK / 4; ++kk) {
             const int row = kk * 4 + lk;
-            double av[4], bv[4];
+            double av[4], bv[NB];
 #pragma unroll
             for (int a = 0; a < 4; ++a) av[a] = sx[(wm * 64 + a * 16 + li) * kGKP + row];
 #pragma unroll
-            for (int b = 0; b < 4; ++b) bv[b] = sy[(wn * 64 + b * 16 + li) * kGKP + row];
+            for (int b = 0; b < NB; ++b) bv[b] = sy[(wn * 16 * NB + b * 16 + li) * kGKP + row];
 #pragma unroll
             for (int a = 0; a < 4; ++a)
 #pragma unroll
-                for (int b = 0; b < 4; ++b) acc[a][b] = mfma(av[a], bv[b], acc[a][b]);
+                for (int b = 0; b < NB; ++b) acc[a][b] = mfma(av[a], bv[b], acc[a][b]);
         }
         __syncthreads();
     }
 #pragma unroll
     for (int a = 0; a < 4; ++a)
 #pragma unroll
-        for (int b = 0; b < 4; ++b)
+        for (int b = 0; b < NB; ++b)
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
                 const int ii = wm * 64 + a * 16 + lk + 4 * v;
-                const int jj = wn * 64 + b * 16 + li;
-                out[ii * kGT + jj] = acc[a][b][v];
+                const int jj = wn * 16 * NB + b * 16 + li;
+                out[ii * TN + jj] = acc[a][b][v];
             }
 }
 
-__global__ __launch_bounds__(kWG) void k_gram_mfma_reduce(int m, int n, int tm, int tn,
-                                                          int nchunk,
+__global__ __launch_bounds__(kWG) void k_gram_mfma_reduce(int m, int n, int tm, int tn, int TM,
+                                                          int TN, int nchunk,
                                                           const double *__restrict__ partial,
                                                           double *__restrict__ G, int ldg)
 {
     // threads walk the partial tiles in storage order, so each chunk's read
     // is contiguous across the wavefront
     const int64_t off = (int64_t)blockIdx.x * kWG + threadIdx.x;
-    const int64_t stride = (int64_t)tm * tn * kGT * kGT;
+    const int64_t stride = (int64_t)tm * tn * TM * TN;
     if (off >= stride) return;
-    const int t = (int)(off / (kGT * kGT));
-    const int within = (int)(off % (kGT * kGT));
-    const int i = (t / tn) * kGT + within / kGT, j = (t % tn) * kGT + within % kGT;
+    const int t = (int)(off / (TM * TN));
+    const int within = (int)(off % (TM * TN));
+    const int i = (t / tn) * TM + within / TN, j = (t % tn) * TN + within % TN;
     if (i >= m || j >= n) return;
     double s = 0.0;
     for (int c = 0; c < nchunk; ++c) s += partial[off + (int64_t)c * stride];
     G[i + (int64_t)j * ldg] = s;
 }
 
+// Tile shape of a Gram of m x n (see k_gram_mfma).
+struct GramShape {
+    int wm, nb;
+    int rows() const { return wm * 64; }
+    int cols() const { return (4 / wm) * 16 * nb; }
+};
+
+GramShape gram_shape(int m, int n)
+{
+    if (m <= 64 && n <= 64) return GramShape{1, 1};
+    if (n <= 64) return GramShape{4, (n + 15) / 16};
+    return GramShape{2, 4};
+}
+
 struct GramPlanM {
+    GramShape sh;
     int tm, tn, nchunk;
     int64_t rows_per_chunk;
+    int64_t tile_doubles() const { return (int64_t)sh.rows() * sh.cols(); }
 };
+
+int64_t gram_tiles(int m, int n)
+{
+    const GramShape sh = gram_shape(m, n);
+    return (int64_t)((m + sh.rows() - 1) / sh.rows()) * ((n + sh.cols() - 1) / sh.cols());
+}
 
 // Row chunking: at most max_wg workgroups in total (256 CUs, two resident
 // each, at the default 512: one more than that runs alone in a second
@@ -195,8 +236,9 @@ struct GramPlanM {
 GramPlanM plan_with(int64_t nrows, int m, int n, int64_t min_rows, int64_t max_wg)
 {
     GramPlanM p;
-    p.tm = (m + kGT - 1) / kGT;
-    p.tn = (n + kGT - 1) / kGT;
+    p.sh = gram_shape(m, n);
+    p.tm = (m + p.sh.rows() - 1) / p.sh.rows();
+    p.tn = (n + p.sh.cols() - 1) / p.sh.cols();
     const int64_t tiles = (int64_t)p.tm * p.tn;
     int64_t want = max_wg / tiles;
     int64_t cap = (nrows + min_rows - 1) / min_rows;
@@ -223,8 +265,9 @@ GramPlanM plan_cand(int64_t nrows, int m, int n, int i)
 
 TuneKey gram_key(int64_t nrows, int m, int n, bool lower_only)
 {
-    const int tiles = ((m + kGT - 1) / kGT) * ((n + kGT - 1) / kGT);
-    return TuneKey{kTuneGram, ilog2_floor(nrows), tiles < 64 ? tiles : 64, lower_only ? 1 : 0};
+    const int64_t tiles = gram_tiles(m, n);
+    return TuneKey{kTuneGram, ilog2_floor(nrows), tiles < 64 ? (int)tiles : 64,
+                   lower_only ? 1 : 0};
 }
 
 
@@ -361,19 +404,20 @@ int gram_mfma_choice(int64_t nrows, int m, int n, bool lower_only, bool *tune)
 static GramPlanM plan_of(int64_t nrows, int m, int n, int choice, int64_t max_chunks)
 {
     if (choice >= 0) return plan_cand(nrows, m, n, choice);
-    const int64_t tiles = (int64_t)((m + kGT - 1) / kGT) * ((n + kGT - 1) / kGT);
-    return plan_with(nrows, m, n, kGramMinRows[0], (max_chunks < 1 ? 1 : max_chunks) * tiles);
+    return plan_with(nrows, m, n, kGramMinRows[0],
+                     (max_chunks < 1 ? 1 : max_chunks) * gram_tiles(m, n));
 }
 
 int64_t gram_mfma_chunk_doubles(int m, int n)
 {
-    return (int64_t)((m + kGT - 1) / kGT) * ((n + kGT - 1) / kGT) * kGT * kGT;
+    const GramShape sh = gram_shape(m, n);
+    return gram_tiles(m, n) * sh.rows() * sh.cols();
 }
 
 int64_t gram_mfma_scratch(int64_t nrows, int m, int n, int choice, int64_t max_chunks)
 {
     const GramPlanM p = plan_of(nrows, m, n, choice, max_chunks);
-    return (int64_t)p.nchunk * p.tm * p.tn * kGT * kGT;
+    return (int64_t)p.nchunk * p.tm * p.tn * p.tile_doubles();
 }
 
 namespace {
@@ -383,11 +427,26 @@ void launch_gram(const GramPlanM &p, int64_t nrows, int m, int n, const double *
                  double *partial, hipStream_t s)
 {
     const int64_t nblk = (int64_t)p.nchunk * p.tm * p.tn;
-    k_gram_mfma<<<(unsigned)nblk, kWG, 0, s>>>(nrows, m, n, X, ldx, Y, ldy, p.tm, p.tn,
-                                               p.rows_per_chunk, lower_only ? 1 : 0, partial);
-    const int64_t total = (int64_t)p.tm * p.tn * kGT * kGT;
+#define GCGE_GRAM(WM, NB)                                                                      \
+    k_gram_mfma<WM, NB><<<(unsigned)nblk, kWG, 0, s>>>(nrows, m, n, X, ldx, Y, ldy, p.tm, p.tn, \
+                                                       p.rows_per_chunk, lower_only ? 1 : 0,   \
+                                                       partial)
+    if (p.sh.wm == 1)
+        GCGE_GRAM(1, 1);
+    else if (p.sh.wm == 4 && p.sh.nb == 1)
+        GCGE_GRAM(4, 1);
+    else if (p.sh.wm == 4 && p.sh.nb == 2)
+        GCGE_GRAM(4, 2);
+    else if (p.sh.wm == 4 && p.sh.nb == 3)
+        GCGE_GRAM(4, 3);
+    else if (p.sh.wm == 4)
+        GCGE_GRAM(4, 4);
+    else
+        GCGE_GRAM(2, 4);
+#undef GCGE_GRAM
+    const int64_t total = (int64_t)p.tm * p.tn * p.tile_doubles();
     k_gram_mfma_reduce<<<(unsigned)((total + kWG - 1) / kWG), kWG, 0, s>>>(
-        m, n, p.tm, p.tn, p.nchunk, partial, G, ldg);
+        m, n, p.tm, p.tn, p.sh.rows(), p.sh.cols(), p.nchunk, partial, G, ldg);
 }
 
 }  // namespace
@@ -404,7 +463,7 @@ void gram_mfma(int64_t nrows, int m, int n, const double *X, int64_t ldx, const 
         int64_t need = 0;
         for (int i = 0; i < kGramCands; ++i) {
             const GramPlanM q = plan_cand(nrows, m, n, i);
-            const int64_t e = (int64_t)q.nchunk * q.tm * q.tn * kGT * kGT;
+            const int64_t e = (int64_t)q.nchunk * q.tm * q.tn * q.tile_doubles();
             if (e > need) need = e;
         }
         double *buf = nullptr;
