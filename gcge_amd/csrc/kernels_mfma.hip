@@ -292,7 +292,9 @@ TuneKey gram_key(int64_t nrows, int m, int n, bool lower_only)
 // problems keep 4.
 constexpr int64_t kRingDeepRows = int64_t(1) << 18;
 
-template <int BM, int kRing>
+// NB: 16-column MFMA tiles per wave (4, a 64-column tile, unless the
+// block is narrower: 40 destination columns run 48 wide instead of 64).
+template <int BM, int kRing, int NB>
 __global__ __launch_bounds__(kWG) void k_lincomb_mfma(
     int64_t nrows, int m, int n, const double *__restrict__ X, int64_t ldx,
     const double *__restrict__ Ct, int ldct, double beta_scalar,
@@ -304,7 +306,7 @@ __global__ __launch_bounds__(kWG) void k_lincomb_mfma(
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int li = lane & 15, lk = lane >> 4;
     const int64_t i0 = tb * 256 + wave * 64;
-    const int j0 = tj * 64;
+    const int j0 = tj * 16 * NB;
     bool rok[4];
     const double *xr[4];
 #pragma unroll
@@ -313,21 +315,21 @@ __global__ __launch_bounds__(kWG) void k_lincomb_mfma(
         rok[a] = row < nrows;
         xr[a] = X + (rok[a] ? row : 0);
     }
-    bool jok[4];
+    bool jok[NB];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) jok[q] = (j0 + q * 16 + li) < n;
+    for (int q = 0; q < NB; ++q) jok[q] = (j0 + q * 16 + li) < n;
     // BM (beta mode) 0: overwrite; 1: scalar beta; 2: per-column beta.  The
     // accumulators start from Y diag(beta), read 16 at a time (a scheduling
     // barrier after each row tile keeps the 64 loads from being in flight at
     // once, which would double the registers and halve the occupancy)
-    d4 acc[4][4];
+    d4 acc[4][NB];
 #pragma unroll
     for (int a = 0; a < 4; ++a) {
         // out-of-range rows and columns read a clamped in-range element and
         // are zeroed by the select (no branch per element)
         const int64_t rowc = rok[a] ? i0 + a * 16 + li : nrows - 1;
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
+        for (int q = 0; q < NB; ++q) {
             d4 init;
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
@@ -348,7 +350,7 @@ __global__ __launch_bounds__(kWG) void k_lincomb_mfma(
     }
     // bx[b][a]: X operand of row tile a, ac[b][q]: C^T operand of column
     // tile q, for the step held in ring buffer b; steps past m load zeros
-    double bx[kRing][4], ac[kRing][4];
+    double bx[kRing][4], ac[kRing][NB];
     // one step into ring buffer B (a macro: the ring index must stay a
     // compile-time constant for the buffers to live in registers)
 #define GCGE_LC_LOAD(B, STEP)                                                         \
@@ -358,7 +360,7 @@ __global__ __launch_bounds__(kWG) void k_lincomb_mfma(
         _Pragma("unroll") for (int a = 0; a < 4; ++a) bx[B][a] =                      \
             (kok_ && rok[a]) ? xr[a][(int64_t)k_ * ldx] : 0.0;                        \
         const double *cr_ = Ct + (int64_t)(kok_ ? k_ : 0) * ldct + j0 + li;           \
-        _Pragma("unroll") for (int q = 0; q < 4; ++q) ac[B][q] =                      \
+        _Pragma("unroll") for (int q = 0; q < NB; ++q) ac[B][q] =                     \
             (kok_ && jok[q]) ? cr_[q * 16] : 0.0;                                     \
     }
 #pragma unroll
@@ -372,7 +374,7 @@ __global__ __launch_bounds__(kWG) void k_lincomb_mfma(
 #pragma unroll
             for (int a = 0; a < 4; ++a)
 #pragma unroll
-                for (int q = 0; q < 4; ++q) acc[a][q] = mfma(ac[u][q], bx[u][a], acc[a][q]);
+                for (int q = 0; q < NB; ++q) acc[a][q] = mfma(ac[u][q], bx[u][a], acc[a][q]);
         }
     }
 #undef GCGE_LC_LOAD
@@ -382,7 +384,7 @@ __global__ __launch_bounds__(kWG) void k_lincomb_mfma(
         if (!rok[a]) continue;
         const int64_t row = i0 + a * 16 + li;
 #pragma unroll
-        for (int q = 0; q < 4; ++q)
+        for (int q = 0; q < NB; ++q)
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
                 const int j = j0 + q * 16 + lk + 4 * v;
@@ -487,25 +489,38 @@ void lincomb_mfma(int64_t nrows, int m, int n, const double *X, int64_t ldx, con
                   int64_t ldy, hipStream_t s)
 {
     if (nrows <= 0 || n <= 0) return;
-    const int ntj = (n + 63) / 64;
+    const int nb = n <= 16 ? 1 : n <= 32 ? 2 : n <= 48 ? 3 : 4;
+    const int ntj = (n + 16 * nb - 1) / (16 * nb);
     const int64_t nrt = (nrows + 255) / 256;
     const unsigned g = (unsigned)(nrt * ntj);
     const bool deep = nrows >= kRingDeepRows;
-#define GCGE_LC_LAUNCH(BM)                                                                  \
+#define GCGE_LC_LAUNCH(BM, NB)                                                              \
     do {                                                                                    \
         if (deep)                                                                           \
-            k_lincomb_mfma<BM, 5><<<g, kWG, 0, s>>>(nrows, m, n, X, ldx, Ct, ldct,          \
-                                                    beta_scalar, beta, Y, ldy, ntj);        \
+            k_lincomb_mfma<BM, 5, NB><<<g, kWG, 0, s>>>(nrows, m, n, X, ldx, Ct, ldct,      \
+                                                        beta_scalar, beta, Y, ldy, ntj);    \
         else                                                                                \
-            k_lincomb_mfma<BM, 4><<<g, kWG, 0, s>>>(nrows, m, n, X, ldx, Ct, ldct,          \
-                                                    beta_scalar, beta, Y, ldy, ntj);        \
+            k_lincomb_mfma<BM, 4, NB><<<g, kWG, 0, s>>>(nrows, m, n, X, ldx, Ct, ldct,      \
+                                                        beta_scalar, beta, Y, ldy, ntj);    \
+    } while (0)
+#define GCGE_LC_NB(BM)                                                                      \
+    do {                                                                                    \
+        if (nb == 1)                                                                        \
+            GCGE_LC_LAUNCH(BM, 1);                                                          \
+        else if (nb == 2)                                                                   \
+            GCGE_LC_LAUNCH(BM, 2);                                                          \
+        else if (nb == 3)                                                                   \
+            GCGE_LC_LAUNCH(BM, 3);                                                          \
+        else                                                                                \
+            GCGE_LC_LAUNCH(BM, 4);                                                          \
     } while (0)
     if (beta_mode == 0)
-        GCGE_LC_LAUNCH(0);
+        GCGE_LC_NB(0);
     else if (beta_mode == 1)
-        GCGE_LC_LAUNCH(1);
+        GCGE_LC_NB(1);
     else
-        GCGE_LC_LAUNCH(2);
+        GCGE_LC_NB(2);
+#undef GCGE_LC_NB
 #undef GCGE_LC_LAUNCH
 }
 
