@@ -159,7 +159,8 @@ def test_linear_comb_aliased(table, mats):
 
 
 @pytest.mark.parametrize("nsd", ["N", "S", "D"])
-@pytest.mark.parametrize("shape", [(5, 5), (64, 64), (130, 70), (1, 1)])
+@pytest.mark.parametrize("shape", [(5, 5), (64, 64), (130, 70), (1, 1), (40, 40), (130, 40),
+                                   (200, 17), (100, 12), (65, 64)])
 def test_inner_prod(table, mats, nsd, shape):
     M, dM = mats["R"]
     n = M.shape[0]
@@ -320,3 +321,85 @@ def test_inner_prod_tall(table, tall, nsd):
     table.destroy_multivec(hx, m + 2)
     if nsd == "N":
         table.destroy_multivec(hy, k)
+
+
+@pytest.mark.parametrize("nsd,m,k", [("N", 200, 40), ("N", 90, 12), ("S", 40, 40), ("S", 64, 64)])
+def test_inner_prod_tall_narrow(table, tall, nsd, m, k):
+    """The Gram's narrow tile shapes at 300,000 rows: a side of at most 64
+    columns (256 x 16 NB tiles) and both sides at most 64 (64 x 64)."""
+    n = 300_000
+    rng = np.random.default_rng(m + k)
+    X = rng.standard_normal((n, m + 2))
+    Y = X.copy() if nsd == "S" else rng.standard_normal((n, k))
+    hx = upload(table, X, tall)
+    hy = hx if nsd == "S" else upload(table, Y, tall)
+    ld = m + 1
+    ip = np.full(ld * k, 99.0)
+    sy = 2 if nsd == "S" else 0
+    table.slot("MultiVecInnerProd")(nsd.encode(), hx, hy, 0, ia(2, sy), ia(2 + m, sy + k),
+                                    ip.ctypes.data_as(abi.c_dbl_p), ld, table.ops)
+    G = O.inner_prod(nsd, X[:, 2:2 + m], (X if nsd == "S" else Y)[:, sy:sy + k])
+    got = ip.reshape((ld, k), order="F")
+    close(got[:m], G)
+    assert np.all(got[m:] == 99.0)
+    table.destroy_multivec(hx, m + 2)
+    if nsd == "N":
+        table.destroy_multivec(hy, k)
+
+
+@pytest.mark.parametrize("mode", ["overwrite", "scalar", "percol"])
+def test_linear_comb_tall_narrow(table, tall, mode):
+    """Y = X C + Y diag(beta) at 300,000 rows into 40 columns (48-wide wave
+    tiles), every beta mode."""
+    n, m, k = 300_000, 150, 40
+    rng = np.random.default_rng(14)
+    X = rng.standard_normal((n, m))
+    Y0 = rng.standard_normal((n, k))
+    coef = np.asfortranarray(rng.standard_normal((m, k)))
+    beta_arr = rng.standard_normal(k)
+    hx, hy = upload(table, X, tall), upload(table, Y0, tall)
+    fn = table.slot("MultiVecLinearComb")
+    s, e = ia(0, 0), ia(m, k)
+    cptr = coef.ctypes.data_as(abi.c_dbl_p)
+    ref = Y0.copy()
+    if mode == "overwrite":
+        fn(hx, hy, 0, s, e, cptr, m, None, 0, table.ops)
+        O.linear_comb(X, ref, coef, None)
+    elif mode == "scalar":
+        b = C.c_double(0.25)
+        fn(hx, hy, 0, s, e, cptr, m, C.byref(b), 0, table.ops)
+        O.linear_comb(X, ref, coef, 0.25)
+    else:
+        fn(hx, hy, 0, s, e, cptr, m, beta_arr.ctypes.data_as(abi.c_dbl_p), 1, table.ops)
+        O.linear_comb(X, ref, coef, beta_arr)
+    close(table.download(hy, 0, k, n), ref)
+    table.destroy_multivec(hx, m)
+    table.destroy_multivec(hy, k)
+
+
+@pytest.mark.parametrize("mode", ["overwrite", "scalar"])
+def test_linear_comb_library_gemm(table, hip, mode):
+    """Past a million rows with at least 100 source and destination columns
+    and a scalar beta, LinearComb is rocBLAS's dgemm: the same results as
+    the oracle at 1,100,000 rows, 110 -> 100 columns."""
+    n, m, k = 1_100_000, 110, 100
+    big = DeviceMatrix(sp.identity(n, format="csc"))
+    rng = np.random.default_rng(15)
+    X = rng.standard_normal((n, m))
+    Y0 = rng.standard_normal((n, k))
+    coef = np.asfortranarray(rng.standard_normal((m + 3, k)))
+    hx, hy = upload(table, X, big), upload(table, Y0, big)
+    fn = table.slot("MultiVecLinearComb")
+    cptr = coef.ctypes.data_as(abi.c_dbl_p)
+    ref = Y0.copy()
+    if mode == "overwrite":
+        fn(hx, hy, 0, ia(0, 0), ia(m, k), cptr, m + 3, None, 0, table.ops)
+        O.linear_comb(X, ref, coef[:m], None)
+    else:
+        b = C.c_double(-1.5)
+        fn(hx, hy, 0, ia(0, 0), ia(m, k), cptr, m + 3, C.byref(b), 0, table.ops)
+        O.linear_comb(X, ref, coef[:m], -1.5)
+    close(table.download(hy, 0, k, n), ref)
+    table.destroy_multivec(hx, m)
+    table.destroy_multivec(hy, k)
+    big.free()
