@@ -641,17 +641,17 @@ void h_MultiVecLinearComb(void **x, void **y, int, int *start, int *end, double 
     MV *X = mv_of(x, "MultiVecLinearComb x");
     check_range(X, start[0], end[0], "MultiVecLinearComb x");
     if (X->nrows != Y->nrows) fail("MultiVecLinearComb: row counts differ", __FILE__, __LINE__);
-    // Over a million rows with at least 100 source and 100 destination
-    // columns and no per-column beta the library GEMM is faster (X C +
-    // beta Y is a plain GEMM; 55-66 TFLOP/s against 45-53 for the
-    // matrix-core kernel at 2-4 M rows, 120-460 source and 120-220
-    // destination columns); otherwise the matrix-core kernel (equal at
-    // 262,144 rows or 320 destination columns, faster below; 35 against 23
-    // TFLOP/s for 40 destination columns at 4 M rows, whose 128-wide tiles
-    // the library fills to a third).  The library takes coef as it is
-    // (column-major, nr rows); the kernel transposed and k-major (Ct[i * nc
-    // + k] = coef[i + k * ldc]).  Coefficients and beta go in one upload.
-    const bool library = Y->nrows >= (int64_t(1) << 20) && nr >= 100 && nc >= 100 &&
+    // From 448 Ki rows with at least 100 source and 100 destination columns
+    // and no per-column beta the library GEMM is faster (X C + beta Y is a
+    // plain GEMM; 47-66 TFLOP/s against 37-53 for the matrix-core kernel
+    // at 0.5-4 M rows, 150-460 source and 100-220 destination columns);
+    // otherwise the matrix-core kernel (equal at 262,144-373,248 rows or
+    // 320 destination columns, faster below; 35 against 23 TFLOP/s for 40
+    // destination columns at 4 M rows, whose 128-wide tiles the library
+    // fills to a third).  The library takes coef as it is (column-major, nr
+    // rows); the kernel transposed and k-major (Ct[i * nc + k] = coef[i + k
+    // * ldc]).  Coefficients and beta go in one upload.
+    const bool library = Y->nrows >= (int64_t(448) << 10) && nr >= 100 && nc >= 100 &&
                          (beta == nullptr || incb == 0) && Y->nrows <= INT32_MAX &&
                          X->ldd <= INT32_MAX && Y->ldd <= INT32_MAX;
     const int64_t ncoef = (int64_t)nr * nc;

@@ -207,7 +207,7 @@ int gcge_hip_device(void);
  *   device BlockPCG, 2: Gram (inner products, both kernels),
  *   3: LinearComb (matrix-core kernel), 4: BLAS-1, 5: vector updates of
  *   the device BlockPCG (p = r + beta p; x += alpha p, r -= alpha w with
- *   r.r), 6: LinearComb by the library GEMM (over a million rows). */
+ *   r.r), 6: LinearComb by the library GEMM (from 448 Ki rows). */
 void gcge_hip_profile_enable(int on);
 void gcge_hip_profile_reset(void);
 void gcge_hip_profile_read(int kind, int64_t *launches, double *total_ms, double *bytes,

@@ -379,7 +379,7 @@ def test_linear_comb_tall_narrow(table, tall, mode):
 
 @pytest.mark.parametrize("mode", ["overwrite", "scalar"])
 def test_linear_comb_library_gemm(table, hip, mode):
-    """Past a million rows with at least 100 source and destination columns
+    """From 448 Ki rows with at least 100 source and destination columns
     and a scalar beta, LinearComb is rocBLAS's dgemm: the same results as
     the oracle at 1,100,000 rows, 110 -> 100 columns."""
     n, m, k = 1_100_000, 110, 100
