@@ -504,6 +504,12 @@ def served(args):
         "max_rel_residual_sampled": worst,
         "queue_s_p50": float(np.percentile([r.queue_seconds for r in res], 50)),
         "solve_s_p50": float(np.percentile([r.solve_seconds for r in res], 50)),
+        # a stream is kept up with when the last request is done within
+        # the arrivals' span plus a median request's latency
+        "arrival_span_s": float(items[-1][0]),
+        "served_requests_per_s": len(items) / wall,
+        "keeps_up": (None if args.config == "cfg4" else
+                     bool(wall <= 1.05 * float(items[-1][0]) + float(np.percentile(lat, 50)))),
         "kernels": kern,
         "roofline": roof, "cpu_baseline": cpu,
         "tuning_entries": tune_entries(),
