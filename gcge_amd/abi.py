@@ -164,6 +164,8 @@ EXPORTS = {
                                                c_int_p, c_dbl_p, C.c_int]),
     "HIP_MultiVecInfo": (None, [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64),
                                 C.POINTER(C.c_int64)]),
+    "HIP_MultiVecCholQR": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p,
+                                     C.c_double, C.c_double]),
     "gcge_hip_set_random_mode": (None, [C.c_int, C.c_uint64]),
     "gcge_hip_synchronize": (None, []),
     "gcge_hip_thread_release": (None, []),
@@ -224,6 +226,7 @@ GCGE_EIG_HOST = 0
 GCGE_EIG_DEVICE = 1
 GCGE_ORTH_MGS = 0
 GCGE_ORTH_CHOLQR = 1
+GCGE_ORTH_CHOLQR_HOST = 2
 GCGE_MODE_FP64 = 0
 GCGE_MODE_MIXED = 1
 GCGE_EPS_RELATIVE = 0

@@ -613,6 +613,39 @@ void h_MultiVecAxpby(double alpha, void **x, double beta, void **y, int *start, 
           ctx().stream);
 }
 
+// Whether X[:, nr] C + Y diag(beta) (nc destination columns) runs as the
+// library GEMM: from 448 Ki rows with at least 100 source and 100
+// destination columns and no per-column beta (X C + beta Y is a plain
+// GEMM; 47-66 TFLOP/s against 37-53 for the matrix-core kernel at 0.5-4 M
+// rows, 150-460 source and 100-220 destination columns); otherwise the
+// matrix-core kernel (equal at 262,144-373,248 rows or 320 destination
+// columns, faster below; 35 against 23 TFLOP/s for 40 destination columns
+// at 4 M rows, whose 128-wide tiles the library fills to a third).
+bool lincomb_library(const MV *X, const MV *Y, int nr, int nc, bool per_column_beta)
+{
+    return Y->nrows >= (int64_t(448) << 10) && nr >= 100 && nc >= 100 && !per_column_beta &&
+           Y->nrows <= INT32_MAX && X->ldd <= INT32_MAX && Y->ldd <= INT32_MAX;
+}
+
+// Y (nrows x nc) = X (nrows x nr) C + Y diag(beta) with C on the device:
+// column-major (nr rows) for the library GEMM, transposed and k-major
+// (C[i * nc + k]) for the matrix-core kernel.
+void lincomb_launch(Ctx &c, const double *X, int64_t ldx, int64_t nrows, int nr, int nc,
+                    const double *dC, bool library, int beta_mode, double beta_scalar,
+                    const double *dbeta, double *Y, int64_t ldy)
+{
+    hipEvent_t ev = c.prof.begin(library ? PROF_GEMM : PROF_LINCOMB,
+                                 8.0 * (double)nrows * (nr + nc * (beta_mode ? 2.0 : 1.0)),
+                                 2.0 * (double)nrows * nr * nc, c.stream);
+    if (library)
+        gemm_nn(nrows, nc, nr, X, ldx, dC, nr, beta_mode == 1 ? beta_scalar : 0.0, Y, ldy);
+    else
+        lincomb(nrows, nr, nc, X, ldx, dC, nc, beta_mode, beta_scalar, dbeta, Y, ldy, c.stream);
+    c.prof.end(ev, c.stream);
+    c.stats.lincomb_calls++;
+    c.stats.lincomb_flops += 2.0 * (double)nrows * nr * nc;
+}
+
 // y = x coef + y diag(beta) (app/app_lapack.c:463-534)
 void h_MultiVecLinearComb(void **x, void **y, int, int *start, int *end, double *coef, int ldc,
                           double *beta, int incb, OPS *)
@@ -641,19 +674,10 @@ void h_MultiVecLinearComb(void **x, void **y, int, int *start, int *end, double 
     MV *X = mv_of(x, "MultiVecLinearComb x");
     check_range(X, start[0], end[0], "MultiVecLinearComb x");
     if (X->nrows != Y->nrows) fail("MultiVecLinearComb: row counts differ", __FILE__, __LINE__);
-    // From 448 Ki rows with at least 100 source and 100 destination columns
-    // and no per-column beta the library GEMM is faster (X C + beta Y is a
-    // plain GEMM; 47-66 TFLOP/s against 37-53 for the matrix-core kernel
-    // at 0.5-4 M rows, 150-460 source and 100-220 destination columns);
-    // otherwise the matrix-core kernel (equal at 262,144-373,248 rows or
-    // 320 destination columns, faster below; 35 against 23 TFLOP/s for 40
-    // destination columns at 4 M rows, whose 128-wide tiles the library
-    // fills to a third).  The library takes coef as it is (column-major, nr
-    // rows); the kernel transposed and k-major (Ct[i * nc + k] = coef[i + k
-    // * ldc]).  Coefficients and beta go in one upload.
-    const bool library = Y->nrows >= (int64_t(448) << 10) && nr >= 100 && nc >= 100 &&
-                         (beta == nullptr || incb == 0) && Y->nrows <= INT32_MAX &&
-                         X->ldd <= INT32_MAX && Y->ldd <= INT32_MAX;
+    // The library takes coef as it is (column-major, nr rows); the kernel
+    // transposed and k-major (Ct[i * nc + k] = coef[i + k * ldc]).
+    // Coefficients and beta go in one upload.
+    const bool library = lincomb_library(X, Y, nr, nc, beta != nullptr && incb != 0);
     const int64_t ncoef = (int64_t)nr * nc;
     Ctx::Staging &g = c.staging(ncoef + nc);
     double *h = g.h;
@@ -678,21 +702,68 @@ void h_MultiVecLinearComb(void **x, void **y, int, int *start, int *end, double 
     double *d = g.d;
     HIPCHECK(hipMemcpyAsync(d, h, sizeof(double) * (ncoef + (beta_mode == 2 ? nc : 0)),
                             hipMemcpyHostToDevice, c.stream));
-    hipEvent_t ev = c.prof.begin(
-        library ? PROF_GEMM : PROF_LINCOMB,
-        8.0 * (double)Y->nrows * (nr + nc * (beta_mode ? 2.0 : 1.0)),
-        2.0 * (double)Y->nrows * nr * nc, c.stream);
-    if (library)
-        gemm_nn(Y->nrows, nc, nr, X->d + (int64_t)start[0] * X->ldd, X->ldd, d, nr,
-                beta_mode == 1 ? beta_scalar : 0.0, ys, Y->ldd);
-    else
-        lincomb(Y->nrows, nr, nc, X->d + (int64_t)start[0] * X->ldd, X->ldd, d, nc, beta_mode,
-                beta_scalar, d + ncoef, ys, Y->ldd, c.stream);
-    c.prof.end(ev, c.stream);
-    c.stats.lincomb_calls++;
-    c.stats.lincomb_flops += 2.0 * (double)Y->nrows * nr * nc;
+    lincomb_launch(c, X->d + (int64_t)start[0] * X->ldd, X->ldd, Y->nrows, nr, nc, d, library,
+                   beta_mode, beta_scalar, d + ncoef, ys, Y->ldd);
     // no wait: the staging slot is reused only after this launch completes
     c.staged(g);
+}
+
+// One pass of block Cholesky QR of x[:, sx:sx+n] in the B inner product
+// (B null: the plain one) with its small dense step on the device: B x into
+// mv_ws[:, 0:n], the Gram G = x^T B x (lower triangle), the coefficients
+// C = D^{-1} R^{-1} from G (cholqr_coef), x C into mv_ws and back into x.
+// The sequence of the host pass -- MultiVecQtAP('S', 'S') with B,
+// Cholesky and triangular inverse on the host, MultiVecLinearComb and
+// MultiVecAxpby(1, ws, 0, x) -- without the Gram's copy to the host; one
+// synchronization reads the factorization's flag at the end.  Returns 1
+// when done, 0 when the block is numerically dependent (x as it was), -1
+// when n is outside 1..1024 (nothing done).
+int cholqr_pass(void **x, int sx, int n, void *matB, void **mv_ws, double zero_tol,
+                double rel_tol)
+{
+    if (n < 1 || n > 1024) return -1;
+    MV *X = mv_of(x, "cholqr x"), *W = mv_of(mv_ws, "cholqr ws");
+    check_range(X, sx, sx + n, "cholqr x");
+    check_range(W, 0, n, "cholqr ws");
+    if (W->nrows != X->nrows) fail("cholqr: row counts differ", __FILE__, __LINE__);
+    Ctx &c = ctx();
+    double *xs = X->d + (int64_t)sx * X->ldd;
+    const double *bx = xs;
+    int64_t ldbx = X->ldd;
+    if (matB != nullptr) {
+        int s[2] = {sx, 0}, e[2] = {sx + n, n};
+        h_MatDotMultiVec(matB, x, mv_ws, s, e, nullptr);
+        bx = W->d;
+        ldbx = W->ldd;
+    }
+    // G, C, the flag and the factorization's work array in one buffer
+    const int64_t nn = (int64_t)n * n;
+    double *G = c.small(2 * nn + 64 + cholqr_coef_work(n));
+    double *C = G + nn, *flag = G + 2 * nn, *work = G + 2 * nn + 64;
+    double *h = c.pinned(1);
+    const GramPlan plan = gram_plan(X->nrows, n, n, true);
+    double *part = c.scratch(plan.scratch);
+    hipEvent_t ev = c.prof.begin(PROF_GRAM, 8.0 * (double)X->nrows * 2.0 * n,
+                                 2.0 * (double)X->nrows * n * n, c.stream);
+    gram(plan, X->nrows, n, n, xs, X->ldd, bx, ldbx, G, n, true, part, c.stream);
+    c.prof.end(ev, c.stream);
+    c.stats.gram_calls++;
+    c.stats.gram_flops += (double)X->nrows * n * n;
+    const bool library = lincomb_library(X, W, n, n, false);
+    if (!cholqr_coef(n, G, n, zero_tol, rel_tol, work, C, library, flag, c.stream))
+        fail("cholqr: order not handled", __FILE__, __LINE__);
+    lincomb_launch(c, xs, X->ldd, X->nrows, n, n, C, library, 0, 0.0, nullptr, W->d, W->ldd);
+    axpby(X->nrows, n, 1.0, W->d, W->ldd, 0.0, xs, X->ldd, c.stream);
+    HIPCHECK(hipMemcpyAsync(h, flag, sizeof(double), hipMemcpyDeviceToHost, c.stream));
+    c.sync();
+    return h[0] == 0.0 ? 1 : 0;
+}
+
+int cholqr_pass_slot(OPS *ops, void **x, int sx, int n, void *matB, void **mv_ws,
+                     double zero_tol, double rel_tol)
+{
+    if (ops == nullptr || ops->MatDotMultiVec != h_MatDotMultiVec) return -1;
+    return cholqr_pass(x, sx, n, matB, mv_ws, zero_tol, rel_tol);
 }
 
 // Inner products to a host array (app/app_lapack.c:299-321, 24-183)
@@ -855,6 +926,11 @@ bool hip_stencil_residual(OPS *ops, void *A, void *B, void **x, int sx, int nc,
                           const double *lam, void **r, int sr)
 {
     return stencil_residual_slot(ops, A, B, x, sx, nc, lam, r, sr);
+}
+int hip_cholqr_pass(OPS *ops, void **x, int sx, int n, void *B, void **mv_ws, double zero_tol,
+                    double rel_tol)
+{
+    return cholqr_pass_slot(ops, x, sx, n, B, mv_ws, zero_tol, rel_tol);
 }
 }  // namespace gcge
 
@@ -1095,6 +1171,12 @@ void gcge_hip_set_random_mode(int mode, uint64_t seed)
     c.rand_mode = mode;
     c.rand_seed = seed;
     c.rand_counter = 0;
+}
+
+int HIP_MultiVecCholQR(void **x, int start, int end, void *B, void **mv_ws, double zero_tol,
+                       double rel_tol)
+{
+    return cholqr_pass(x, start, end - start, B, mv_ws, zero_tol, rel_tol);
 }
 
 void gcge_hip_synchronize(void) { ctx().sync(); }

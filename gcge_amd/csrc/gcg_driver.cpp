@@ -36,6 +36,8 @@ int64_t pcg32_iterations(OPS *ops);
 // operators of one grid (false otherwise); app_hip.cpp.
 bool hip_stencil_residual(OPS *ops, void *A, void *B, void **x, int sx, int nc,
                           const double *lam, void **r, int sr);
+int hip_cholqr_pass(OPS *ops, void **x, int sx, int n, void *B, void **mv_ws, double zero_tol,
+                    double rel_tol);
 }
 
 namespace {
@@ -69,7 +71,7 @@ struct Solver {
     int compRR_min_num = -1;
     double compRR_min_gap = 0.01, compRR_tol = 1e-16;
     int eig_backend = -1;  // -1: automatic
-    int orth_method = 0;   // GCGE_ORTH_MGS (the reference's) or GCGE_ORTH_CHOLQR
+    int orth_method = 0;   // GCGE_ORTH_MGS (the reference's), GCGE_ORTH_CHOLQR(_HOST)
     // run state
     OPS *ops = nullptr;
     void *A = nullptr, *B = nullptr;
@@ -228,8 +230,14 @@ bool cholesky_upper(int n, const std::vector<double> &G, std::vector<double> &R,
 // B-orthogonality: a 16^3 solve stalled on spurious near-zero Ritz values
 // and a 48^3 solve returned vectors with max|X^T B X - I| = 2e-8.  It costs
 // one Gram and one LinearComb per block (1.5 % of the 128^3 mixed solve).
+//
+// With device set and ops the HIP table, each pass's Gram, factorization,
+// inverse and product stay on the device (hip_cholqr_pass): the same
+// tests and the same X D^{-1} R^{-1}, with the factorization's own
+// rounding (right-looking, R^{-1} from the inverse of R^T by forward
+// substitution).
 void cholqr(void **x, int start_x, int *end_x, void *B, const OrthParams &p, void **mv_ws,
-            OPS *ops, int64_t *fallbacks)
+            OPS *ops, int64_t *fallbacks, bool device)
 {
     if (*end_x <= start_x) return;
     // two block passes ("twice is enough" for classical Gram-Schmidt: the
@@ -241,6 +249,14 @@ void cholqr(void **x, int start_x, int *end_x, void *B, const OrthParams &p, voi
     std::vector<double> G((size_t)n * n), R, Rinv((size_t)n * n), dinv((size_t)n);
     for (int pass = 0; pass < 2; ++pass) {
         if (pass == 1) remove_previous(x, start_x, *end_x, B, p, mv_ws, ops, 1);
+        const int done =
+            device ? gcge::hip_cholqr_pass(ops, x, start_x, n, B, mv_ws, p.zero_tol, 1e-10) : -1;
+        if (done == 0) {
+            *fallbacks += 1;
+            mgs_blocks(x, start_x, end_x, B, p, mv_ws, ops);
+            return;
+        }
+        if (done == 1) continue;
         int s[2] = {start_x, start_x}, e[2] = {*end_x, *end_x};
         ops->MultiVecQtAP('S', 'S', x, B, x, 0, s, e, G.data(), n, mv_ws, ops);
         bool ok = true;
@@ -481,10 +497,11 @@ int nV(const Solver &S) { return S.sizeV - S.sizeC; }
 // selected method (the reference always uses MGS)
 void orthonormalize(Solver &S, void **V, int start, int *end, const OrthParams &p, void **ws)
 {
-    if (S.orth_method == GCGE_ORTH_CHOLQR)
+    if (S.orth_method == GCGE_ORTH_CHOLQR || S.orth_method == GCGE_ORTH_CHOLQR_HOST)
     {
         S.cholqr_blocks += 1;
-        cholqr(V, start, end, S.B, p, ws, S.ops, &S.cholqr_fallbacks);
+        cholqr(V, start, end, S.B, p, ws, S.ops, &S.cholqr_fallbacks,
+               S.orth_method == GCGE_ORTH_CHOLQR);
     }
     else
         mgs(V, start, end, S.B, p, ws, S.ops);
@@ -899,7 +916,7 @@ void gcg(void *A, void *B, double *eval, void **evec, int nevGiven, int *nevConv
     } while (numIter < numIterMax);
     S.numIter = numIter + (S.numIterMax - numIterMax);
     for (int i = 0; i < S.sizeX; ++i) eval[i] = S.ss_eval[i];
-    if (S.orth_method == GCGE_ORTH_CHOLQR) final_rayleigh_quotients(S, ritz, *nevConv, eval, V);
+    if (S.orth_method != GCGE_ORTH_MGS) final_rayleigh_quotients(S, ritz, *nevConv, eval, V);
     if (S.printtime) {
         double tot = 0.0;
         for (int k : {T_INITX, T_CHECK, T_COMPP, T_RR, T_RV, T_COMPW, T_COMPX}) tot += S.times[k];

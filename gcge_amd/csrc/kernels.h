@@ -120,6 +120,17 @@ void gram_mfma(int64_t nrows, int m, int n, const double *X, int64_t ldx, const 
 void lincomb_mfma(int64_t nrows, int m, int n, const double *X, int64_t ldx, const double *Ct,
                   int ldct, int beta_mode, double beta_scalar, const double *beta, double *Y,
                   int64_t ldy, hipStream_t s);
+// Block Cholesky QR coefficients from the lower triangle of G = X^T B X
+// (n x n, ldg): C = D^{-1} R^{-1} with D = diag(G)^{1/2} and
+// D^{-1} G D^{-1} = R^T R, column-major (C[i + n k]) when colmajor, else
+// transposed and k-major (C[i n + k]) as lincomb takes it.  flag[0] = 1
+// and C = I when a column's B-norm is below zero_tol or a pivot is not
+// above rel_tol times the largest scaled diagonal entry, else 0.  work
+// holds cholqr_coef_work(n) doubles.  False (nothing launched) unless
+// 1 <= n <= 1024.
+int64_t cholqr_coef_work(int n);
+bool cholqr_coef(int n, const double *G, int ldg, double zero_tol, double rel_tol,
+                 double *work, double *coef, bool colmajor, double *flag, hipStream_t s);
 // U[0,1) from SplitMix64(seed, element index) into an nrows x ncols block
 // whose first element has global index offset.
 void random_uniform(int64_t nrows, int ncols, double *Y, int64_t ldy, uint64_t seed,

@@ -17,6 +17,9 @@ import numpy as np
 
 from . import abi, problems
 
+ORTH_METHODS = {"mgs": abi.GCGE_ORTH_MGS, "cholqr": abi.GCGE_ORTH_CHOLQR,
+                "cholqr_host": abi.GCGE_ORTH_CHOLQR_HOST}
+
 PHASES = ("initX", "checkconv", "compP", "compRR", "rr_matW", "dsyevx", "compRV",
           "compW", "compX", "linsol")
 
@@ -150,8 +153,10 @@ def solve(A, B, nev: int, block_size: int, nev_init: int, nev_max: int,
     solver, src/ops_eig_sol_gcg.c:503).
     rand "libc": initial block from srand(seed)/rand() as the reference;
     "device": counter-based generator on the device.
-    orth "mgs" (the reference's orthogonalization) or "cholqr" (block
-    Cholesky QR); default "mgs" for fp64 and "cholqr" for mixed.
+    orth "mgs" (the reference's orthogonalization), "cholqr" (block
+    Cholesky QR with its factorization on the device) or "cholqr_host"
+    (the factorization on the host); default "mgs" for fp64 and "cholqr"
+    for mixed.
     graphs: replay the mixed path's BlockPCG iteration groups as captured
     HIP graphs (False launches every kernel eagerly).
     `mats` may pass pre-uploaded (DeviceMatrix A, DeviceMatrix B).
@@ -196,8 +201,7 @@ def solve(A, B, nev: int, block_size: int, nev_init: int, nev_max: int,
                                        else abi.GCGE_EIG_DEVICE)
     if orth is None:
         orth = "cholqr" if mode == "mixed" else "mgs"
-    lib.HIPGCG_SetOrthMethod(ops, abi.GCGE_ORTH_CHOLQR if orth == "cholqr"
-                             else abi.GCGE_ORTH_MGS)
+    lib.HIPGCG_SetOrthMethod(ops, ORTH_METHODS[orth])
     if mode == "mixed":
         lib.HIPGCG_MultiLinearSolverSetup_BlockPCG32(30, 1e-2, 1e-14, hB, 0.0, ops)
         if not graphs:
@@ -275,8 +279,7 @@ def solve_host_dense(A: np.ndarray, B: np.ndarray, nev: int, block_size: int, ne
                                 0, 0, mv_ws, None, None, ops)
     lib.HIPGCG_Setparameters(gap_min, shift, ops)
     lib.HIPGCG_SetDenseEigensolver(ops, abi.GCGE_EIG_HOST)
-    lib.HIPGCG_SetOrthMethod(ops, abi.GCGE_ORTH_CHOLQR if orth == "cholqr"
-                             else abi.GCGE_ORTH_MGS)
+    lib.HIPGCG_SetOrthMethod(ops, ORTH_METHODS[orth])
     ev = np.zeros(nev_max)
     nconv = C.c_int(nev)
     t0 = time.perf_counter()

@@ -139,6 +139,20 @@ void HIP_MultiVecGetToHost(void **mv, int start, int end, double *dst, int64_t l
 /* Rows, columns and leading dimension of a multivector handle. */
 void HIP_MultiVecInfo(void **mv, int64_t *nrows, int64_t *ncols, int64_t *ldd);
 
+/* One Cholesky QR pass over columns [start, end) of x in the B inner
+ * product (B NULL: the Euclidean one), all on the device: G = X^T B X,
+ * D = diag(G)^{1/2}, D^{-1} G D^{-1} = R^T R, X <- X D^{-1} R^{-1};
+ * mv_ws (at least end - start columns, same rows) is overwritten.  The
+ * block-orthonormalization step the driver's GCGE_ORTH_CHOLQR runs twice
+ * per block in place of the column-by-column self-orthonormalization of
+ * the reference's OrthSelf (src/ops_orth.c:271-368).  Returns 1 when
+ * done; 0, with x unchanged, when a column's B-norm is below zero_tol or a
+ * pivot of the scaled factorization is not above rel_tol (times the
+ * largest scaled diagonal entry); -1 when the block is empty or wider
+ * than 1024 columns (nothing done). */
+int HIP_MultiVecCholQR(void **x, int start, int end, void *B, void **mv_ws, double zero_tol,
+                       double rel_tol);
+
 /* Allocation-free inner products.  No counterpart in the reference: this
  * is for callers that cannot allocate device memory (code that must not
  * call hipMalloc, such as tight serving loops or harnesses that forbid
@@ -310,9 +324,14 @@ int gcge_host_dense_eigh(int n, const double *A, int lda, double *w, double *Z, 
  * orthonormalizes the block itself by Cholesky QR applied twice, falling
  * back to MGS when the block is numerically rank deficient.  Same
  * subspace, different rounding: results agree in eigenvalues and
- * residuals, not bit for bit. */
+ * residuals, not bit for bit.  With the HIP table, GCGE_ORTH_CHOLQR keeps
+ * the Gram matrix, its Cholesky factorization and the triangular inverse
+ * on the device (blocks of up to 1024 columns; wider ones, and other
+ * tables, factor on the host); GCGE_ORTH_CHOLQR_HOST always factors on
+ * the host. */
 #define GCGE_ORTH_MGS 0
 #define GCGE_ORTH_CHOLQR 1
+#define GCGE_ORTH_CHOLQR_HOST 2
 void HIPGCG_SetOrthMethod(OPS *ops, int which);
 /* Counts of the last solve: blocks orthonormalized with GCGE_ORTH_CHOLQR,
  * and how many of them fell back to MGS. */
