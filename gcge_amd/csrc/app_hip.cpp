@@ -759,6 +759,85 @@ int cholqr_pass(void **x, int sx, int n, void *matB, void **mv_ws, double zero_t
     return h[0] == 0.0 ? 1 : 0;
 }
 
+// The P step of GCG (src/ops_eig_sol_gcg.c ComputeP) with its small dense
+// work on the device: the coefficient columns Z[:, sp:sp+np] (host, n rows,
+// ld n) are orthonormalized against Z[:, 0:sp] and among themselves in the
+// identity inner product -- the earlier columns removed twice, then
+// Cholesky QR twice with one more removal between the passes, the scheme
+// of the block orthonormalization of W -- and written back into Z, and
+// ws[:, 0:np] = V[:, sv:sv+n] times them is copied into V[:, dst:dst+np].
+// One synchronization reads both passes' flags and the columns before the
+// product.  Returns 1 when done, 0 when a pass finds the columns
+// numerically dependent (V, ws and Z untouched: the caller orthonormalizes
+// Z on the host), -1 when ops is not this table or np is outside 1..1024
+// (nothing done).
+int orth_coef_comb_slot(OPS *ops, double *Z, int n, int sp, int np, double zero_tol,
+                        void **v, int sv, void **ws, int dst)
+{
+    if (ops == nullptr || ops->MatDotMultiVec != h_MatDotMultiVec || np < 1 || np > 1024 ||
+        n < 1 || sp < 0)
+        return -1;
+    MV *V = mv_of(v, "ComputeP V"), *W = mv_of(ws, "ComputeP ws");
+    check_range(V, sv, sv + n, "ComputeP V");
+    check_range(V, dst, dst + np, "ComputeP V");
+    check_range(W, 0, np, "ComputeP ws");
+    if (W->nrows != V->nrows) fail("ComputeP: row counts differ", __FILE__, __LINE__);
+    Ctx &c = ctx();
+    // device copy of Z[:, 0:sp+np]; the staging slot stays in use until the
+    // event recorded after the last launch that reads it
+    const int64_t nz = (int64_t)n * (sp + np);
+    Ctx::Staging &g = c.staging(nz);
+    memcpy(g.h, Z, sizeof(double) * nz);
+    HIPCHECK(hipMemcpyAsync(g.d, g.h, sizeof(double) * nz, hipMemcpyHostToDevice, c.stream));
+    const double *Zx = g.d;
+    double *Zp = g.d + (int64_t)n * sp;
+    // T (n x np), Cx (sp x np), G and C (np x np), the coefficient
+    // transpose (np x n), the flags and the factorization's work array
+    const int64_t nT = (int64_t)n * np, nC = (int64_t)np * np;
+    double *T = c.small(2 * nT + (int64_t)sp * np + 2 * nC + 64 + cholqr_coef_work(np));
+    double *Ft = T + nT, *Cx = Ft + nT, *G = Cx + (int64_t)sp * np, *C = G + nC,
+           *flag = C + nC, *work = flag + 64;
+    double *h = c.pinned(2);
+    auto remove_previous = [&](double *Y) {
+        if (sp == 0) return;
+        gemm_small(true, sp, np, n, 1.0, Zx, n, Y, n, 0.0, Cx, sp);
+        gemm_small(false, n, np, sp, -1.0, Zx, n, Cx, sp, 1.0, Y, n);
+    };
+    auto pass = [&](const double *Y, double *out, double *f) {
+        gemm_small(true, np, np, n, 1.0, Y, n, Y, n, 0.0, G, np);
+        if (!cholqr_coef(np, G, np, zero_tol, 1e-10, work, C, true, f, c.stream))
+            fail("ComputeP: order not handled", __FILE__, __LINE__);
+        gemm_small(false, n, np, np, 1.0, Y, n, C, np, 0.0, out, n);
+    };
+    remove_previous(Zp);
+    remove_previous(Zp);
+    pass(Zp, T, flag);
+    remove_previous(T);
+    pass(T, Zp, flag + 1);
+    // the orthonormalized columns go back to Z too: the next Rayleigh-Ritz
+    // step projects with them on the host (P^T A P from the coefficients)
+    HIPCHECK(hipMemcpyAsync(g.h + (int64_t)n * sp, Zp, sizeof(double) * nT,
+                            hipMemcpyDeviceToHost, c.stream));
+    HIPCHECK(hipMemcpyAsync(h, flag, 2 * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+    c.sync();
+    if (h[0] != 0.0 || h[1] != 0.0) {
+        c.staged(g);
+        return 0;
+    }
+    memcpy(Z + (int64_t)n * sp, g.h + (int64_t)n * sp, sizeof(double) * nT);
+    const bool library = lincomb_library(V, W, n, np, false);
+    const double *coef = Zp;
+    if (!library) {
+        transpose(n, np, Zp, n, Ft, np);
+        coef = Ft;
+    }
+    lincomb_launch(c, V->d + (int64_t)sv * V->ldd, V->ldd, V->nrows, n, np, coef, library, 0, 0.0,
+                   nullptr, W->d, W->ldd);
+    axpby(V->nrows, np, 1.0, W->d, W->ldd, 0.0, V->d + (int64_t)dst * V->ldd, V->ldd, c.stream);
+    c.staged(g);
+    return 1;
+}
+
 int cholqr_pass_slot(OPS *ops, void **x, int sx, int n, void *matB, void **mv_ws,
                      double zero_tol, double rel_tol)
 {
@@ -931,6 +1010,11 @@ int hip_cholqr_pass(OPS *ops, void **x, int sx, int n, void *B, void **mv_ws, do
                     double rel_tol)
 {
     return cholqr_pass_slot(ops, x, sx, n, B, mv_ws, zero_tol, rel_tol);
+}
+int hip_orth_coef_comb(OPS *ops, double *Z, int n, int sp, int np, double zero_tol,
+                       void **V, int sv, void **ws, int dst)
+{
+    return orth_coef_comb_slot(ops, Z, n, sp, np, zero_tol, V, sv, ws, dst);
 }
 }  // namespace gcge
 

@@ -220,6 +220,26 @@ void gemm_nn(int64_t m, int n, int k, const double *A, int64_t lda, const double
     if (st != rocblas_status_success) fail("rocblas_dgemm failed", __FILE__, __LINE__);
 }
 
+void gemm_small(bool transa, int m, int n, int k, double alpha, const double *A, int lda,
+                const double *B, int ldb, double beta, double *C, int ldc)
+{
+    DevEig &d = deveig();
+    const rocblas_status st = rocblas_dgemm(
+        d.h, transa ? rocblas_operation_transpose : rocblas_operation_none,
+        rocblas_operation_none, m, n, k, &alpha, A, lda, B, ldb, &beta, C, ldc);
+    if (st != rocblas_status_success) fail("rocblas_dgemm failed", __FILE__, __LINE__);
+}
+
+void transpose(int m, int n, const double *A, int lda, double *C, int ldc)
+{
+    DevEig &d = deveig();
+    const double one = 1.0, zero = 0.0;
+    const rocblas_status st =
+        rocblas_dgeam(d.h, rocblas_operation_transpose, rocblas_operation_transpose, n, m, &one,
+                      A, lda, &zero, A, lda, C, ldc);
+    if (st != rocblas_status_success) fail("rocblas_dgeam failed", __FILE__, __LINE__);
+}
+
 int eigh_device(int n, double *A, int lda, double *w, double *Z, int ldz)
 {
     if (n <= 0) return 0;

@@ -38,6 +38,8 @@ bool hip_stencil_residual(OPS *ops, void *A, void *B, void **x, int sx, int nc,
                           const double *lam, void **r, int sr);
 int hip_cholqr_pass(OPS *ops, void **x, int sx, int n, void *B, void **mv_ws, double zero_tol,
                     double rel_tol);
+int hip_orth_coef_comb(OPS *ops, double *Z, int n, int sp, int np, double zero_tol,
+                       void **V, int sv, void **ws, int dst);
 }
 
 namespace {
@@ -626,6 +628,20 @@ void compute_p(Solver &S, void **V, const std::vector<int> &offset)
                    sizeof(double) * (o2 - o1));
     }
     int startP = S.sizeX - S.sizeC, endP = startP + S.sizeP;
+    // With the block orthonormalization on the device, coefficient blocks of
+    // 4 M multiply-adds and more (order 460 by 120 columns: 19 M) are
+    // orthonormalized there too, by Cholesky QR instead of the MGS below
+    // (same span; an MGS on the host costs ~5 ms at that size, the device
+    // steps ~0.6 ms); dependent columns fall back to the MGS.
+    if (S.orth_method == GCGE_ORTH_CHOLQR && S.sizeP > 0 && S.endW - S.startN == n &&
+        (double)n * S.sizeP * endP >= 4e6 &&
+        gcge::hip_orth_coef_comb(ops, Z, n, startP, S.sizeP, S.orthP.zero_tol, V, S.startN,
+                                 S.mv_ws[1], startP + S.sizeC) == 1) {
+        S.startP = startP + S.sizeC;
+        S.endP = endP + S.sizeC;
+        S.sizeP = S.endP - S.startP;
+        return;
+    }
     // small MGS of the P coefficients in the identity inner product
     gcge_dense_mv vecP{Z, n, endP, n};
     std::vector<double> wsbuf((size_t)n * (S.sizeP > 0 ? S.sizeP : 1));

@@ -132,3 +132,25 @@ def test_solve_device_and_host_factorization_agree(hip, mode, n, nev):
         assert np.abs(G - np.eye(r.nev_conv)).max() <= 1e-10, o
     a, b = res["cholqr"].eval[:prob.nev], res["cholqr_host"].eval[:prob.nev]
     assert (np.abs(a - b) / np.abs(b)).max() <= 1e-10
+
+
+def test_solve_device_compute_p(hip):
+    """A 32^3 solve with config 2's block sizes (nev 100, block 120, 220
+    initial vectors), whose P coefficient blocks (order ~460 by 120) are
+    orthonormalized on the device by Cholesky QR: eigenvalues within 1e-10
+    of the analytic ones and of the run that orthonormalizes them on the
+    host by MGS, B-orthonormal eigenvectors."""
+    prob = P.stencil_problem(32, 3, 100, 120, 220, 220, name="3d_32")
+    res = {o: S.solve_problem(prob, shift=0, mode="mixed", want_vectors=True, orth=o)
+           for o in ("cholqr", "cholqr_host")}
+    exact = P.analytic_eigenvalues(32, 3, prob.nev)
+    for o, r in res.items():
+        assert r.nev_conv >= prob.nev, o
+        lam = r.eval[:prob.nev]
+        assert (np.abs(lam - exact) / exact).max() <= 1e-10, o
+        X = r.evec[:, :r.nev_conv]
+        G = X.T @ (prob.B @ X)
+        assert np.abs(G - np.eye(r.nev_conv)).max() <= 1e-10, o
+    assert res["cholqr"].phases["compP"] < res["cholqr_host"].phases["compP"]
+    a, b = res["cholqr"].eval[:prob.nev], res["cholqr_host"].eval[:prob.nev]
+    assert (np.abs(a - b) / np.abs(b)).max() <= 1e-10
