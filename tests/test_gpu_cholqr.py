@@ -8,7 +8,9 @@ the result against B-orthonormality.  Widths cover the LDS factorization
 (up to 128 columns) and the global-memory one (129 to 1024), and both
 layouts of the coefficients: transposed for the matrix-core linear
 combination, column-major for the library GEMM (from 448 Ki rows and 100
-columns).  A numerically dependent block must leave X bit for bit as it
+columns), and the three forms of the factorization kernel (4 waves up to
+64 columns, 16 waves in LDS up to 128, global memory above).  A
+numerically dependent block must leave X bit for bit as it
 was, so the caller's fallback to Gram-Schmidt starts from the same state.
 """
 
@@ -66,7 +68,8 @@ def run_pass(table, hip, X, B, dB, start, extra=2):
 
 
 @pytest.mark.parametrize("N,n,withB", [
-    (1000, 1, True), (5000, 7, True), (20000, 40, False), (20000, 120, True),
+    (1000, 1, True), (5000, 7, True), (20000, 40, False), (20000, 64, True),
+    (20000, 65, False), (20000, 120, True),
     (20000, 128, True), (20000, 129, True), (30000, 300, False),
     (460000, 100, True), (460000, 130, True)])
 def test_cholqr_pass_matches_numpy(table, hip, N, n, withB):
