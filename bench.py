@@ -453,7 +453,7 @@ def served(args):
     lib = abi.load()
     lib.gcge_hip_profile_all(1)  # every worker's and the batcher's launches, timed
     with Service(workers=args.workers, hbm_budget=args.hbm_budget,
-                 batching=args.batching == "on") as svc:
+                 batching=args.batching == "on", orth=args.service_orth) as svc:
         # warm the workers' kernels and handles on a small problem each
         warm = [svc.submit(P.stencil_problem(24, 2, 8), seed=1) for _ in range(args.workers)]
         for t in warm:
@@ -495,6 +495,7 @@ def served(args):
         "vs_baseline": None, "dtype": "f64+f32", "data": "synthetic",
         "config": {"workload": what, "path": "mixed", "workers": args.workers,
                    "batched_inner_solves": args.batching == "on",
+                   "orthonormalization": args.service_orth,
                    "hbm_budget_bytes": args.hbm_budget, "parallelism": "single GPU"},
         "batched_rounds": bst,
         "latency_ms": {"p50": 1e3 * float(np.percentile(lat, 50)),
@@ -534,6 +535,8 @@ def main():
     ap.add_argument("--count", type=int, default=512, help="cfg4/cfg5 requests")
     ap.add_argument("--qps", type=float, default=20.0, help="cfg5 arrival rate")
     ap.add_argument("--workers", type=int, default=16, help="cfg4/cfg5 service workers")
+    ap.add_argument("--service-orth", default="cholqr_host", choices=("cholqr", "cholqr_host"),
+                    help="cfg4/cfg5 mixed requests: Cholesky QR factorized on the device or host")
     ap.add_argument("--batching", default="on", choices=("on", "off"),
                     help="cfg4/cfg5: batched inner solves across requests")
     ap.add_argument("--hbm-budget", type=float, default=200e9)

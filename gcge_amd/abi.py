@@ -206,6 +206,7 @@ EXPORTS = {
     "gcge_service_stats": (None, [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int),
                                   C.POINTER(C.c_int64)]),
     "gcge_service_set_batching": (None, [C.c_void_p, C.c_int]),
+    "gcge_service_set_orth": (C.c_int, [C.c_void_p, C.c_int]),
     "gcge_service_batch_stats": (None, [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64),
                                         c_int_p]),
     "gcge_service_destroy": (None, [C.c_void_p]),

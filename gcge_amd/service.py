@@ -41,12 +41,18 @@ def _ccs(M):
 
 
 class Service:
-    def __init__(self, workers: int = 8, hbm_budget: float = 200e9, batching: bool = True):
+    def __init__(self, workers: int = 8, hbm_budget: float = 200e9, batching: bool = True,
+                 orth: str = "cholqr_host"):
         self.lib = abi.load()
         self.h = self.lib.gcge_service_create(int(workers), int(hbm_budget))
         if not self.h:
             raise ValueError("gcge_service_create rejected its arguments")
         self.lib.gcge_service_set_batching(self.h, 1 if batching else 0)
+        # "cholqr_host" (factorizations on the host, the default) or "cholqr"
+        if self.lib.gcge_service_set_orth(
+                self.h, {"cholqr": abi.GCGE_ORTH_CHOLQR,
+                         "cholqr_host": abi.GCGE_ORTH_CHOLQR_HOST}[orth]) != 0:
+            raise ValueError(orth)
         self._live = {}
 
     @staticmethod

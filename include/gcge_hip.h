@@ -422,6 +422,12 @@ void gcge_service_destroy(void *svc);
  * after the call.  Stats: batched rounds, request iterations they carried,
  * and the most requests one round carried. */
 void gcge_service_set_batching(void *svc, int on);
+/* Block orthonormalization of the mixed-precision requests started after
+ * the call: GCGE_ORTH_CHOLQR_HOST (the default: Cholesky factorizations on
+ * the host, which serve many concurrent small requests faster) or
+ * GCGE_ORTH_CHOLQR (on the device).  Returns 0, or -1 for another value.
+ * No counterpart in the reference (its solver has one orthogonalization). */
+int gcge_service_set_orth(void *svc, int method);
 void gcge_service_batch_stats(void *svc, int64_t *rounds, int64_t *solve_iterations,
                               int *max_solves);
 
