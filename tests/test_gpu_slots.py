@@ -160,7 +160,8 @@ def test_linear_comb_aliased(table, mats):
 
 @pytest.mark.parametrize("nsd", ["N", "S", "D"])
 @pytest.mark.parametrize("shape", [(5, 5), (64, 64), (130, 70), (1, 1), (40, 40), (130, 40),
-                                   (200, 17), (100, 12), (65, 64)])
+                                   (200, 17), (100, 12), (65, 64), (40, 1), (130, 2), (17, 4),
+                                   (3, 100)])
 def test_inner_prod(table, mats, nsd, shape):
     M, dM = mats["R"]
     n = M.shape[0]
@@ -323,10 +324,13 @@ def test_inner_prod_tall(table, tall, nsd):
         table.destroy_multivec(hy, k)
 
 
-@pytest.mark.parametrize("nsd,m,k", [("N", 200, 40), ("N", 90, 12), ("S", 40, 40), ("S", 64, 64)])
+@pytest.mark.parametrize("nsd,m,k", [("N", 200, 40), ("N", 90, 12), ("S", 40, 40), ("S", 64, 64),
+                                     ("N", 40, 1), ("N", 130, 2), ("N", 33, 8)])
 def test_inner_prod_tall_narrow(table, tall, nsd, m, k):
     """The Gram's narrow tile shapes at 300,000 rows: a side of at most 64
-    columns (256 x 16 NB tiles) and both sides at most 64 (64 x 64)."""
+    columns (256 x 16 NB tiles) and both sides at most 64 (64 x 64), and
+    the thin stream for a side of at most 8 (16 X columns per workgroup
+    for one or two Y columns, several column tiles)."""
     n = 300_000
     rng = np.random.default_rng(m + k)
     X = rng.standard_normal((n, m + 2))
