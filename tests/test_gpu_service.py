@@ -62,6 +62,28 @@ def test_same_alone_and_in_the_mix(hip, mode):
         assert np.array_equal(a.eval, m.eval), p.name
 
 
+def test_device_factorization_option(hip):
+    """Mixed requests with their Cholesky QR factorizations on the device
+    (gcge_service_set_orth(GCGE_ORTH_CHOLQR)) instead of the default host:
+    the same bounds, the same requests alone and in the mix bit for bit,
+    and eigenvalues within 1e-10 of the host-factorized runs."""
+    with Service(workers=1) as svc:
+        host = [svc.wait(svc.submit(p, mode="mixed", seed=11)) for p in PROBS[:4]]
+    with Service(workers=1, orth="cholqr") as svc:
+        alone = [svc.wait(svc.submit(p, mode="mixed", seed=11, want_vectors=True))
+                 for p in PROBS[:4]]
+    with Service(workers=3, orth="cholqr") as svc:
+        mix = [svc.wait(t) for t in [svc.submit(p, mode="mixed", seed=11, want_vectors=True)
+                                     for p in PROBS[:4]]]
+    for p, h, a, m in zip(PROBS, host, alone, mix):
+        check(p, a)
+        assert np.array_equal(a.eval, m.eval), p.name
+        k = min(h.nev_conv, a.nev_conv)
+        assert np.max(np.abs(a.eval[:k] - h.eval[:k]) / np.abs(h.eval[:k])) <= 1e-10
+    with Service(workers=1) as svc:
+        assert svc.lib.gcge_service_set_orth(svc.h, 0) == -1
+
+
 def test_admission_by_footprint(hip):
     fps = [Service.footprint(p) for p in PROBS[:4]]
     budget = int(1.5 * max(fps))          # room for one request at a time
