@@ -3,8 +3,7 @@
 // triangle), the coefficients C = D^{-1} R^{-1} of X <- X C, where
 // D = diag(G)^{1/2} and D^{-1} G D^{-1} = R^T R.  One workgroup; the
 // factorization is right-looking Cholesky of L = R^T in LDS (orders up to
-// 128; up to 64 in a 4-wave form) or in a global work array (up to 1024),
-// one barrier per column, and
+// 128) or in a global work array (up to 1024), one barrier per column, and
 // R^{-1} = (L^{-1})^T by column-oriented forward substitution, one column
 // of L^{-1} per wavefront held in registers (the pivot entry goes to every
 // lane by readlane), so the substitution needs no barrier and no LDS
@@ -28,7 +27,6 @@ namespace gcge {
 namespace {
 
 constexpr int kCqThreads = 1024;            // 16 waves
-constexpr int kCqSmall = 64;                // orders up to this: 4 waves, 33 KB of LDS
 constexpr int kCqLds = 128;                 // orders factored in LDS
 constexpr int kCqMax = 1024;
 
@@ -167,12 +165,7 @@ bool cholqr_coef(int n, const double *G, int ldg, double zero_tol, double rel_to
                  double *work, double *coef, bool colmajor, double *flag, hipStream_t s)
 {
     if (n < 1 || n > kCqMax || ldg < n) return false;
-    // the small form leaves room on a CU for other streams' workgroups (the
-    // 128 form's 133 KB of LDS needs a CU of its own)
-    if (n <= kCqSmall)
-        k_cholqr_coef<kCqSmall, true, 256><<<1, 256, 0, s>>>(
-            n, G, ldg, zero_tol, rel_tol, nullptr, coef, colmajor ? 1 : 0, flag);
-    else if (n <= kCqLds)
+    if (n <= kCqLds)
         k_cholqr_coef<kCqLds, true, kCqThreads><<<1, kCqThreads, 0, s>>>(
             n, G, ldg, zero_tol, rel_tol, nullptr, coef, colmajor ? 1 : 0, flag);
     else

@@ -8,9 +8,9 @@ the result against B-orthonormality.  Widths cover the LDS factorization
 (up to 128 columns) and the global-memory one (129 to 1024), and both
 layouts of the coefficients: transposed for the matrix-core linear
 combination, column-major for the library GEMM (from 448 Ki rows and 100
-columns), and the three forms of the factorization kernel (4 waves up to
-64 columns, 16 waves in LDS up to 128, global memory above).  A
-numerically dependent block must leave X bit for bit as it
+columns), and both forms of the factorization kernel (LDS up to 128
+columns, global memory above).  A numerically dependent block must leave X
+bit for bit as it
 was, so the caller's fallback to Gram-Schmidt starts from the same state.
 """
 
